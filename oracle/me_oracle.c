@@ -1,0 +1,1453 @@
+/*
+ * me_oracle.c — CPU FP64 oracle of the Milne–Eddington inversion path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the
+ * checker and by bench.py's cpu_baseline leg.  The product never links it.
+ *
+ * Each routine restates the reference's arithmetic in the reference's own
+ * operation order (so that the Faddeeva and SVD parts agree bit for bit with
+ * the reference built from its sources into oracle/_ref), with the reference
+ * line ranges it follows cited at each function.  How every function is
+ * pinned is listed in me_oracle.h.
+ */
+#include "me_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI      3.14159265358979323846
+#define ORC_SQRTPI  1.77245385090551588191
+#define ORC_C       299792458.0
+
+/* ======================================================================== */
+/* Faddeeva w(z), z = nu + i y: H = Re w, L = Im w  (Src/FADDEEVA.c)         */
+/* ======================================================================== */
+
+/* Rybicki's Dawson integral as in Src/FADDEEVA.c:29-79 */
+static double orc_dawson(double x)
+{
+    const double A1 = 2.0 / 3.0, A2 = 0.4, A3 = 2.0 / 7.0, HH = 0.4;
+    double ax = fabs(x);
+    if (ax < 0.2) {
+        double x2 = x * x;
+        return x * (1.0 - A1 * x2 * (1.0 - A2 * x2 * (1.0 - A3 * x2)));
+    }
+    double cc[7];
+    for (int i = 1; i <= 6; i++) {
+        double t = (2.0 * i - 1.0) * HH;
+        cc[i] = exp(-t * t);
+    }
+    int n0 = 2 * (int)round(ax / (2 * HH));
+    double xp = ax - n0 * HH;
+    double e1 = exp(2.0 * xp * HH);
+    double e2 = e1 * e1;
+    double d1 = n0 + 1;
+    double d2 = d1 - 2.0;
+    double acc = 0.0;
+    for (int i = 1; i <= 6; i++, d1 += 2.0, d2 -= 2.0, e1 *= e2)
+        acc += cc[i] * (e1 / d1 + 1.0 / (d2 * e1));
+    double t = exp(-xp * xp);
+    return 0.5641895835 * copysign(t, x) * acc;
+}
+
+/* erfcx(a) = exp(a^2) erfc(a), asymptotic series above 26.6 (Src/FADDEEVA.c:83-108) */
+static double orc_erfcx(double a)
+{
+    if (a > 26.6) {
+        double q = a * a;
+        return ((((((162.421875 / q - 29.53125) / q + 6.5625) / q - 1.875) / q + 0.75) / q - 0.5) / q + 1)
+               / ORC_SQRTPI / a;
+    }
+    return exp(a * a) * erfc(a);
+}
+
+/* Hui, Armstrong & Wray (1978) p=6 rational form (Src/FADDEEVA.c:112-163) */
+static void orc_hui6(double nu, double y, double *H, double *L)
+{
+    static const double P[7] = {122.607931777104326, 214.382388694706425, 181.928533092181549,
+                                93.155580458138441, 30.180142196210589, 5.912626209773153,
+                                0.564189583562615};
+    static const double Q[7] = {122.60793177387535, 352.730625110963558, 457.334478783897737,
+                                348.703917719495792, 170.354001821091472, 53.992906912940207,
+                                10.479857114260399};
+    double zr = y, zi = -nu;
+    double nr = P[6], ni = 0.0;
+    double dr = Q[6] + zr, di = zi;
+    for (int k = 5; k >= 0; k--) {
+        double tr = nr * zr - ni * zi + P[k];
+        double ti = nr * zi + ni * zr;
+        nr = tr; ni = ti;
+        tr = dr * zr - di * zi + Q[k];
+        ti = dr * zi + di * zr;
+        dr = tr; di = ti;
+    }
+    double den = dr * dr + di * di;
+    *H = (nr * dr + ni * di) / den;
+    *L = (ni * dr - nr * di) / den;
+}
+
+/* Humlíček (1982) W4, used for nfigures <= 5 (Src/FADDEEVA.c:167-281) */
+static void orc_hum4(double nu, double y, double *H, double *L)
+{
+    const double anu = fabs(nu);
+    const double S = anu + y;
+    const double tr = y, ti = -nu;
+    if (S >= 15) {
+        const double ur = tr * tr - ti * ti, ui = 2.0 * tr * ti;
+        double dr = 0.5 + ur, di = ui;
+        double den = dr * dr + di * di;
+        *H = 0.56418958355 * (tr * dr + ti * di) / den;
+        *L = 0.56418958355 * (ti * dr - tr * di) / den;
+    } else if (S >= 5.5) {
+        const double ur = tr * tr - ti * ti, ui = 2.0 * tr * ti;
+        double nr = 1.4104739589 + 0.5641896 * ur, ni = 0.5641896 * ui;
+        double dr = 0.75 + 3.0 * ur + (ur * ur - ui * ui);
+        double di = 3.0 * ui + 2.0 * ur * ui;
+        double den = dr * dr + di * di;
+        double qr = (nr * dr + ni * di) / den;
+        double qi = (ni * dr - nr * di) / den;
+        *H = tr * qr - ti * qi;
+        *L = tr * qi + ti * qr;
+    } else if (y >= 1.95 * anu - 0.176) {
+        const double c[5] = {0.5642236, 3.778987, 11.96482, 20.20933, 16.4955};
+        const double d[6] = {1.0, 6.699398, 21.69274, 39.27121, 38.82363, 16.4955};
+        double nr = c[0], ni = 0, dr = d[0], di = 0, xr, xi;
+        for (int k = 1; k < 5; k++) {
+            xr = nr * tr - ni * ti + c[k];
+            xi = nr * ti + ni * tr;
+            nr = xr; ni = xi;
+        }
+        for (int k = 1; k < 6; k++) {
+            xr = dr * tr - di * ti + d[k];
+            xi = dr * ti + di * tr;
+            dr = xr; di = xi;
+        }
+        double den = dr * dr + di * di;
+        *H = (nr * dr + ni * di) / den;
+        *L = (ni * dr - nr * di) / den;
+    } else {
+        const double c[7] = {0.5641900381, 1.320521697, 35.7668278, 219.0312964, 1540.786893,
+                             3321.990492, 36183.30536};
+        const double d[8] = {1, 1.841438936, 61.57036588, 364.2190727, 2186.181081, 9022.227659,
+                             24322.84021, 32066.59372};
+        const double ur = tr * tr - ti * ti, ui = 2.0 * tr * ti;
+        double eu = exp(ur);
+        double er = eu * cos(ui), ei = eu * sin(ui);
+        double nr = c[0], ni = 0, dr = d[0], di = 0, xr, xi;
+        for (int k = 1; k < 7; k++) {
+            xr = -(nr * ur - ni * ui) + c[k];
+            xi = -(nr * ui + ni * ur);
+            nr = xr; ni = xi;
+        }
+        xr = nr * tr - ni * ti;
+        xi = nr * ti + ni * tr;
+        nr = xr; ni = xi;
+        for (int k = 1; k < 8; k++) {
+            xr = -(dr * ur - di * ui) + d[k];
+            xi = -(dr * ui + di * ur);
+            dr = xr; di = xi;
+        }
+        double den = dr * dr + di * di;
+        *H = er - (nr * dr + ni * di) / den;
+        *L = ei - (ni * dr - nr * di) / den;
+    }
+}
+
+/* Zaghloul & Ali (2011) Algorithm 916 as in Src/FADDEEVA.c:286-402 */
+static void orc_alg916(double nu, double y, const orc_fadd *F, double *H, double *L)
+{
+    if (nu == 0) { *H = orc_erfcx(y); *L = 0; return; }
+    if (y == 0) { *H = exp(-nu * nu); *L = 2.0 / ORC_SQRTPI * orc_dawson(nu); return; }
+
+    const double x = fabs(nu), ya = fabs(y);
+    if (x + ya > 1e7) {
+        if (x < ya) {
+            double r = x / ya;
+            *H = 1. / ORC_SQRTPI / (r * r + 1.);
+            *L = r / ORC_SQRTPI / (r * r + 1.);
+        } else {
+            double r = ya / x;
+            *H = r / ORC_SQRTPI / (r * r + 1.);
+            *L = 1. / ORC_SQRTPI / (r * r + 1.);
+        }
+        return;
+    }
+    const double a = F->a;
+    const double xx = x * x, yy = y * y, xy = x * y;
+    const double xy2 = 2 * xy;
+    const double sxy = sin(xy), s2xy = sin(xy2), c2xy = cos(xy2);
+    const double exx = exp(-xx), e2ax = exp(2 * a * x);
+    const double k1 = exx * orc_erfcx(y);
+
+    /* the reference truncates sqrt of a negative number to INT_MIN (x86) */
+    double t1 = F->log_rmin - x * x;
+    int nmax1 = (t1 >= 0) ? (int)(sqrt(t1) / a + 0.5) : INT_MIN;
+    int nmax2 = (int)((F->sqrt_log_rmin - x) / a + 0.5);
+    int nmx = nmax1 > nmax2 ? nmax1 : nmax2;
+    if (nmx >= F->nmax) nmx = F->nmax - 1;
+
+    int nb = (int)(x / a + 0.5);
+    int nf = nb + 1;
+    const double dxf = a * nf - x;
+    const double dxb = x - a * nb;
+    int nmf = (int)((F->sqrt_log_rmin - dxf) / a);
+    int nmb = (int)((F->sqrt_log_rmin - dxb) / a);
+    if (nmf >= F->nmax) nmf = F->nmax - 1;
+    if (nmb >= F->nmax) nmb = F->nmax - 1;
+    const int n_lo = (nb - nmb) > 1 ? (nb - nmb) : 1;
+
+    double re = k1 * c2xy + 2 * a * x * sxy * exx * sxy / xy / M_PI;
+    double im = -k1 * s2xy + 2 * a * x * exx * s2xy / xy2 / M_PI;
+    double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, ep, d;
+
+    double ea = exp(-dxf * dxf);
+    double eb = exp(-2 * a * dxf);
+    ep = 1.;
+    for (int n = nf; n <= nf + nmf; n++) {
+        d = ea * F->e[n - nf] / (F->aa * n * n + yy);
+        s3 += d * ep;
+        s5 += d * ep * a * n;
+        ep *= eb;
+    }
+    ea = exp(-dxb * dxb);
+    eb = exp(-2 * a * dxb);
+    ep = 1.;
+    for (int n = nb; n >= n_lo; n--) {
+        d = ea * F->e[nb - n] / (F->aa * n * n + yy);
+        s3 += d * ep;
+        s5 += d * ep * a * n;
+        ep *= eb;
+    }
+    re += a * y * s3 / M_PI;
+    im += a * s5 / M_PI;
+
+    if (x < F->sqrt_log_rmin || x < 10) {
+        ep = 1.;
+        for (int n = 1; n <= nmx; n++) {
+            d = exx * F->e[n] / (F->aa * n * n + yy);
+            s1 += d;
+            ep /= e2ax;
+            s2 += d * ep;
+            s4 += d * ep * a * n;
+        }
+        re += a * y * (-2 * c2xy * s1 + s2) / M_PI;
+        im += a * (2 * y * s2xy * s1 - s4) / M_PI;
+    }
+    *H = re;
+    *L = nu > 0 ? im : -im;
+}
+
+/* constants of Algorithm 916 for nfigures digits (Src/FADDEEVA.c:406-436) */
+void orc_fadd_init(orc_fadd *F, int nfigures)
+{
+    F->nfig = nfigures;
+    F->rmin = pow(10., -nfigures);
+    F->a = M_PI / sqrt(-log(0.5 * F->rmin));
+    F->aa = F->a * F->a;
+    F->log_rmin = -log(F->rmin);
+    F->sqrt_log_rmin = sqrt(F->log_rmin);
+    F->nmax = 50;
+    F->ifig = nfigures > 4 ? nfigures - 4 : 0;
+    if (F->ifig > 4) F->ifig = 4;
+    F->e[0] = 1.;
+    for (int i = 1; i < F->nmax; i++) F->e[i] = exp(-F->aa * i * i);
+}
+
+/* region thresholds on |z|^2 by accuracy (Src/FADDEEVA.c:471-477) */
+static const double ORC_THR[5][5] = {{1.6e4, 160.0, 107.0, 28.5, 3.5},
+                                     {1.5e5, 510.0, 110.0, 39., -0.1},
+                                     {1.451e6, 1.6e3, 180.0, 110.0, -0.1},
+                                     {1.5e7, 5.01e3, 380.0, 115.0, 114.0},
+                                     {1.3e8, 1.6e4, 810.0, 195.0, 116.0}};
+
+/* dispatcher of Src/FADDEEVA.c:440-590; returns the branch id */
+int orc_faddeeva(double nu, double y, const orc_fadd *F, double *H, double *L)
+{
+    if (nu == 0) { *H = orc_erfcx(y); *L = 0; return ORC_FNU0; }
+    if (y == 0) { *H = exp(-nu * nu); *L = 2.0 / ORC_SQRTPI * orc_dawson(nu); return ORC_FY0; }
+
+    const double *T = ORC_THR[F->ifig];
+    const double zr = nu, zi = y;
+    const double z2r = zr * zr - zi * zi;
+    const double z2i = 2.0 * zr * zi;
+    const double r2 = zr * zr + zi * zi;
+    const double y2 = y * y;
+
+    if (r2 >= T[0]) {
+        double den = ORC_SQRTPI * (zr * zr + zi * zi);
+        *H = zi / den;
+        *L = zr / den;
+        return ORC_FAS1;
+    }
+    if (r2 >= T[1]) {
+        double wr = z2r - 0.5;
+        double den = ORC_SQRTPI * (wr * wr + z2i * z2i);
+        *H = (-zi * wr + zr * z2i) / den;
+        *L = (zr * wr + zi * z2i) / den;
+        return ORC_FAS2;
+    }
+    if (r2 >= T[2]) {
+        double wr = z2r - 1.5;
+        double pr = wr * zr - z2i * zi;
+        double pi = wr * zi + z2i * zr;
+        double den = ORC_SQRTPI * (pr * pr + pi * pi);
+        wr = z2r - 1.0;
+        *H = (-z2i * pr + wr * pi) / den;
+        *L = (wr * pr + z2i * pi) / den;
+        return ORC_FAS3;
+    }
+    if (r2 >= T[3]) {
+        if (F->nfig <= 4 && y2 < 6e-14) { orc_hum4(nu, y, H, L); return ORC_FHUM; }
+        if (F->nfig == 5 && r2 < 39 && y2 < 1e-9) { orc_hum4(nu, y, H, L); return ORC_FHUM; }
+        double wr = z2r - 2.5;
+        double ar = -(wr * zi + z2i * zr);
+        double ai = wr * zr - z2i * zi;
+        wr = z2r - 3.;
+        double br = wr * z2r - z2i * z2i + 0.75;
+        double bi = (wr + z2r) * z2i;
+        double den = ORC_SQRTPI * (br * br + bi * bi);
+        *H = (ar * br + ai * bi) / den;
+        *L = (-ar * bi + ai * br) / den;
+        return ORC_FAS4;
+    }
+    if (r2 >= T[4]) {
+        if (F->nfig <= 4) {
+            if (y2 < 0.026) { orc_hum4(nu, y, H, L); return ORC_FHUM; }
+            orc_hui6(nu, y, H, L);
+            return ORC_FHUI;
+        } else if (F->nfig == 5) {
+            if (y2 >= 0.27) { orc_hui6(nu, y, H, L); return ORC_FHUI; }
+            if (y2 >= 1e-9) { orc_alg916(nu, y, F, H, L); return ORC_F916; }
+            orc_hum4(nu, y, H, L);
+            return ORC_FHUM;
+        } else if (F->nfig == 6) {
+            if (y2 >= 1) { orc_hui6(nu, y, H, L); return ORC_FHUI; }
+            orc_alg916(nu, y, F, H, L);
+            return ORC_F916;
+        }
+        double wr = z2r - 4.5;
+        double ar = -(wr + z2r) * z2i;
+        double ai = wr * z2r - z2i * z2i + 2.;
+        wr = z2r - 5.;
+        double br = wr * z2r - z2i * z2i + 3.75;
+        double bi = (wr + z2r) * z2i;
+        double cr = br * zr - bi * zi;
+        double ci = br * zi + bi * zr;
+        double den = ORC_SQRTPI * (cr * cr + ci * ci);
+        *H = (ar * cr + ai * ci) / den;
+        *L = (-ar * ci + ai * cr) / den;
+        return ORC_FAS4;
+    }
+    if (F->nfig <= 4) { orc_hui6(nu, y, H, L); return ORC_FHUI; }
+    orc_alg916(nu, y, F, H, L);
+    return ORC_F916;
+}
+
+void orc_faddeeva_vec(int64_t n, const double *nu, const double *y, int nfigures, double *H,
+                      double *L, int *branch)
+{
+    orc_fadd F;
+    orc_fadd_init(&F, nfigures);
+    for (int64_t i = 0; i < n; i++) {
+        int b = orc_faddeeva(nu[i], y[i], &F, &H[i], &L[i]);
+        if (branch) branch[i] = b;
+    }
+}
+
+/* ======================================================================== */
+/* Golub–Reinsch SVD (Golub & Reinsch 1970, EISPACK svd), with the sign and  */
+/* ordering conventions of Src/SVD_TEMPLATE.h:84-344; U overwrites A.        */
+/* ======================================================================== */
+
+#define ORC_SVD_MAXIT 50
+
+/* sqrt(a^2+b^2) as in Src/SVD_TEMPLATE.h:49-80 */
+static inline double orc_hyp(double a, double b)
+{
+    double aa = a > 0 ? a : -a, ab = b > 0 ? b : -b;
+    double mx = aa > ab ? aa : ab;
+    if (mx == 0) return 0;
+    if (mx < 1e150) return sqrt(a * a + b * b);
+    double mn = aa > ab ? ab : aa;
+    double r = mn / mx;
+    return mx * sqrt(1.0 + r * r);
+}
+
+int orc_svd(double *A, int m, int n, double *w, double *V)
+{
+#define AM(i, j) A[(i) * n + (j)]
+#define VM(i, j) V[(i) * n + (j)]
+    double e[64];
+    double g = 0., scale = 0., anorm = 0.;
+    int l = 0, fail = 0;
+
+    /* a non-finite matrix would send the split search below past e[0]; the
+     * reference has no guard (undefined behaviour there) — report failure */
+    for (int i = 0; i < m * n; i++) {
+        if (!isfinite(A[i])) {
+            for (int k = 0; k < m * n; k++) A[k] = 0.0;
+            for (int k = 0; k < n * n; k++) V[k] = (k % (n + 1)) == 0 ? 1.0 : 0.0;
+            for (int k = 0; k < n; k++) w[k] = 0.0;
+            return 2;
+        }
+    }
+
+    /* 1. Householder reduction to upper bidiagonal form */
+    for (int i = 0; i < n; i++) {
+        l = i + 1;
+        e[i] = scale * g;
+        double s = 0.0;
+        g = scale = 0.0;
+        if (i < m) {
+            for (int k = i; k < m; k++) scale += fabs(AM(k, i));
+            if (scale) {
+                for (int k = i; k < m; k++) {
+                    AM(k, i) /= scale;
+                    s += AM(k, i) * AM(k, i);
+                }
+                double f = AM(i, i);
+                g = f >= 0 ? -sqrt(s) : sqrt(s);
+                double h = f * g - s;
+                AM(i, i) = f - g;
+                for (int j = l; j < n; j++) {
+                    double dot = 0.0;
+                    for (int k = i; k < m; k++) dot += AM(k, i) * AM(k, j);
+                    double fct = dot / h;
+                    for (int k = i; k < m; k++) AM(k, j) += fct * AM(k, i);
+                }
+                for (int k = i; k < m; k++) AM(k, i) *= scale;
+            }
+        }
+        w[i] = scale * g;
+        s = 0.0;
+        g = scale = 0.0;
+        if (i < m && i != n - 1) {
+            for (int k = l; k < n; k++) scale += fabs(AM(i, k));
+            if (scale) {
+                for (int k = l; k < n; k++) {
+                    AM(i, k) /= scale;
+                    s += AM(i, k) * AM(i, k);
+                }
+                double f = AM(i, l);
+                g = f >= 0 ? -sqrt(s) : sqrt(s);
+                double h = f * g - s;
+                AM(i, l) = f - g;
+                for (int k = l; k < n; k++) e[k] = AM(i, k) / h;
+                for (int j = l; j < m; j++) {
+                    double dot = 0.0;
+                    for (int k = l; k < n; k++) dot += AM(j, k) * AM(i, k);
+                    for (int k = l; k < n; k++) AM(j, k) += dot * e[k];
+                }
+                for (int k = l; k < n; k++) AM(i, k) *= scale;
+            }
+        }
+        double t = fabs(w[i]) + fabs(e[i]);
+        if (anorm < t) anorm = t;
+    }
+
+    /* 2. accumulate the right-hand transformations into V */
+    for (int i = n - 1; i >= 0; i--) {
+        if (i < n - 1) {
+            if (g != 0.0) {
+                for (int j = l; j < n; j++) VM(j, i) = (AM(i, j) / AM(i, l)) / g;
+                for (int j = l; j < n; j++) {
+                    double dot = 0.0;
+                    for (int k = l; k < n; k++) dot += AM(i, k) * VM(k, j);
+                    for (int k = l; k < n; k++) VM(k, j) += dot * VM(k, i);
+                }
+            }
+            for (int j = l; j < n; j++) VM(i, j) = VM(j, i) = 0.0;
+        }
+        VM(i, i) = 1.0;
+        g = e[i];
+        l = i;
+    }
+
+    /* 3. accumulate the left-hand transformations into A (becomes U) */
+    for (int i = (m <= n ? m : n) - 1; i >= 0; i--) {
+        l = i + 1;
+        g = w[i];
+        for (int j = l; j < n; j++) AM(i, j) = 0.0;
+        if (g != 0.0) {
+            g = 1.0 / g;
+            for (int j = l; j < n; j++) {
+                double dot = 0.0;
+                for (int k = l; k < m; k++) dot += AM(k, i) * AM(k, j);
+                double f = (dot / AM(i, i)) * g;
+                for (int k = i; k < m; k++) AM(k, j) += f * AM(k, i);
+            }
+            for (int j = i; j < m; j++) AM(j, i) *= g;
+        } else {
+            for (int j = i; j < m; j++) AM(j, i) = 0.0;
+        }
+        AM(i, i) += 1.0;
+    }
+
+    /* 4. diagonalise the bidiagonal form by implicit-shift QR */
+    for (int k = n - 1; k >= 0; k--) {
+        for (int its = 0; its < ORC_SVD_MAXIT; its++) {
+            int split = 1, nm = 0;
+            for (l = k; l >= 0; l--) {
+                nm = l - 1;
+                if ((fabs(e[l]) + anorm) == anorm) { split = 0; break; }
+                if (nm >= 0 && (fabs(w[nm]) + anorm) == anorm) break;
+            }
+            if (split) {
+                /* cancel e[l] when w[l-1] is negligible */
+                double c = 0.0, s = 1.0;
+                for (int i = l; i <= k; i++) {
+                    double f = s * e[i];
+                    e[i] = c * e[i];
+                    if ((fabs(f) + anorm) == anorm) break;
+                    double gg = w[i];
+                    double h = orc_hyp(f, gg);
+                    w[i] = h;
+                    h = 1.0 / h;
+                    c = gg * h;
+                    s = -f * h;
+                    for (int j = 0; j < m; j++) {
+                        double yv = AM(j, nm), zv = AM(j, i);
+                        AM(j, nm) = yv * c + zv * s;
+                        AM(j, i) = zv * c - yv * s;
+                    }
+                }
+            }
+            double z = w[k];
+            if (l == k) {
+                if (z < 0.0) {
+                    w[k] = -z;
+                    for (int j = 0; j < n; j++) VM(j, k) = -VM(j, k);
+                }
+                break;
+            }
+            if (its == ORC_SVD_MAXIT - 1) fail = 1;
+            double x = w[l];
+            nm = k - 1;
+            double yv = w[nm];
+            double gg = e[nm];
+            double h = e[k];
+            double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) / (2.0 * h * yv);
+            gg = orc_hyp(f, 1.0);
+            double sg = f >= 0.0 ? gg : -gg;
+            f = ((x - z) * (x + z) + h * (yv / (f + sg) - h)) / x;
+            double c = 1.0, s = 1.0;
+            for (int j = l; j <= nm; j++) {
+                int i = j + 1;
+                gg = e[i];
+                yv = w[i];
+                h = s * gg;
+                gg *= c;
+                z = orc_hyp(f, h);
+                e[j] = z;
+                c = f / z;
+                s = h / z;
+                f = x * c + gg * s;
+                gg = gg * c - x * s;
+                h = yv * s;
+                yv *= c;
+                for (int jj = 0; jj < n; jj++) {
+                    double xv = VM(jj, j), zv = VM(jj, i);
+                    VM(jj, j) = xv * c + zv * s;
+                    VM(jj, i) = zv * c - xv * s;
+                }
+                z = orc_hyp(f, h);
+                w[j] = z;
+                if (z) {
+                    z = 1.0 / z;
+                    c = f * z;
+                    s = h * z;
+                }
+                f = c * gg + s * yv;
+                x = c * yv - s * gg;
+                for (int jj = 0; jj < m; jj++) {
+                    double y2 = AM(jj, j), z2 = AM(jj, i);
+                    AM(jj, j) = y2 * c + z2 * s;
+                    AM(jj, i) = z2 * c - y2 * s;
+                }
+            }
+            e[l] = 0.0;
+            e[k] = f;
+            w[k] = x;
+        }
+    }
+    return fail;
+#undef AM
+#undef VM
+}
+
+/* x = V diag(1/w, 0 below 1e-50) U^T b  (Src/SVD_TEMPLATE.h:348-406) */
+void orc_svbksb(const double *u, const double *v, const double *w, int m, int n,
+                const double *b, double *x)
+{
+    double t[64];
+    for (int j = 0; j < n; j++) {
+        double s = 0.0;
+        if (w[j] > 1e-50) {
+            for (int i = 0; i < m; i++) s += u[i * n + j] * b[i];
+            s /= w[j];
+        }
+        t[j] = s;
+    }
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += v[i * n + j] * t[j];
+        x[i] = s;
+    }
+}
+
+/* ======================================================================== */
+/* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11)                        */
+/* ======================================================================== */
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+#define ORC_STREAM_JUMP  0x4A554D00u
+#define ORC_STREAM_TRUTH 0x54525554u
+#define ORC_STREAM_NOISE 0x4E4F4953u
+
+static inline double orc_u01(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
+/* two Box–Muller pairs from one Philox block */
+static void orc_gauss4(uint64_t seed, int64_t pix, uint32_t c2, uint32_t stream, double g[4])
+{
+    uint32_t ctr[4] = {(uint32_t)pix, (uint32_t)((uint64_t)pix >> 32), c2, stream};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    orc_philox(ctr, key, o);
+    for (int p = 0; p < 2; p++) {
+        double u1 = orc_u01(o[2 * p]), u2 = orc_u01(o[2 * p + 1]);
+        double r = sqrt(-2.0 * log(u1));
+        double t = 2.0 * ORC_PI * u2;
+        g[2 * p] = r * cos(t);
+        g[2 * p + 1] = r * sin(t);
+    }
+}
+
+/* ======================================================================== */
+/* context: INV_INIT derived constants (Src/INV_INIT.c:121-205)              */
+/* ======================================================================== */
+
+orc_ctx *orc_ctx_create(const me_config *cfg, const double *lambda, int nl)
+{
+    if (nl < 2 || nl > 512 || !cfg) return NULL;
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->cfg = *cfg;
+    c->nl = nl;
+    c->lambda = (double *)malloc(sizeof(double) * nl);
+    memcpy(c->lambda, lambda, sizeof(double) * nl);
+    c->bshift = 4.6686e-10 * cfg->geff * cfg->lambda0 * cfg->lambda0;
+    c->npar = 0;
+    for (int p = 0; p < 9; p++) {
+        if (!cfg->fixed[p]) c->npar++;
+        c->limits[p][0] = cfg->bounds[p][0];
+        c->limits[p][1] = cfg->bounds[p][1];
+    }
+    c->ncut = nl / 5;
+    double t = (lambda[c->ncut] - cfg->lambda0) * ORC_C / 1e3;
+    if (c->limits[3][0] < t) c->limits[3][0] = t;
+    t = (lambda[nl - 1 - c->ncut] - cfg->lambda0) * ORC_C / 1e3;
+    if (c->limits[3][1] > t) c->limits[3][1] = t;
+
+    /* H4: the comparison yields 0/1, so ic is the last index below lambda0 */
+    c->ic = 0;
+    for (int iw = 0; iw < nl - 1; iw++) {
+        double b = cfg->lambda0 > lambda[iw];
+        double p = lambda[iw + 1] * cfg->lambda0;
+        if (b > 0 && p > 0) c->ic = b < p ? iw : iw + 1;
+    }
+    int h1 = 2, h2 = 2;
+    if (c->ic - c->ncut >= 2) h1 = c->ic - c->ncut;
+    if (nl - 1 - c->ncut - c->ic >= 2) h2 = nl - 1 - c->ncut - c->ic;
+    int half = h1 > h2 ? h1 : h2;
+    c->i0 = c->ic - half;
+    if (c->i0 < 0) { c->i0 = 0; half = c->ic - c->i0; }
+    c->i1 = c->ic + half;
+    if (c->i1 > nl - 1) { c->i1 = nl - 1; half = c->i1 - c->ic; c->i0 = c->ic - half; }
+
+    for (int s = 0; s < 4; s++) c->wsq[s] = cfg->weights[s] * cfg->weights[s];
+    orc_fadd_init(&c->F, cfg->nfigures);
+    c->regul_any = 0;
+    for (int p = 0; p < 9; p++)
+        if (cfg->regul[p]) c->regul_any = 1;
+    return c;
+}
+
+void orc_ctx_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    free(c->lambda);
+    free(c);
+}
+
+void orc_ctx_info(const orc_ctx *c, int *ic, int *i0, int *i1, int *ncut, double *vlos)
+{
+    *ic = c->ic; *i0 = c->i0; *i1 = c->i1; *ncut = c->ncut;
+    vlos[0] = c->limits[3][0];
+    vlos[1] = c->limits[3][1];
+}
+
+/* ======================================================================== */
+/* Milne–Eddington synthesis with analytic Jacobian (Src/ME_SOLVER.c:101-366) */
+/* ======================================================================== */
+
+void orc_synth(const orc_ctx *c, const double *par, const double *step, double *syn, double *jac,
+               int64_t *fcount)
+{
+    const int nl = c->nl;
+    const double lam0 = c->cfg.lambda0;
+    const double dw = par[4], damp = par[5], eta0 = par[6], hEta = 0.5 * eta0;
+    const double S = par[7], beta = par[8];
+    const double s0 = S * beta, s1 = S - s0;
+    const double bnorm = c->bshift / dw;
+    const double lnorm = lam0 / dw;
+    const double st = sin(par[1]), ct = cos(par[1]);
+    const double st2 = st * st, ct2p1 = ct * ct + 1;
+    const double c2p = cos(2. * par[2]), s2p = sin(2. * par[2]);
+    const double kQ = hEta * st2 * c2p, kU = hEta * st2 * s2p;
+    const double kT = hEta * 2.0 * st * ct, kV = hEta * ct;
+    const double shB = par[0] * bnorm;
+    const double shV = 1e3 * (-1e3 * par[3] / ORC_C * lam0 - lam0) / dw;
+
+    double *oI = syn, *oQ = syn + nl, *oU = syn + 2 * nl, *oV = syn + 3 * nl;
+
+    for (int iw = 0; iw < nl; iw++) {
+        double shW = 1e3 * c->lambda[iw] / dw + shV;
+        double ph[3], ps[3];
+        double dph[3][6], dps[3][6];
+        for (int q = -1; q <= 1; q++) {
+            int k = q + 1;
+            double v = shW + q * shB;
+            double H, L;
+            int b = orc_faddeeva(v, damp, &c->F, &H, &L);
+            if (fcount) fcount[b < 8 ? b : ORC_FHUI]++;
+            ph[k] = H / ORC_SQRTPI;
+            ps[k] = L / ORC_SQRTPI;
+            if (jac) {
+                double dH = 2. * (-v * H + damp * L) / ORC_SQRTPI;
+                double dL = -2. * (-1. / ORC_SQRTPI + damp * H + v * L) / ORC_SQRTPI;
+                dph[k][0] = q * bnorm * dH;
+                dps[k][0] = q * bnorm * dL;
+                dph[k][3] = -1e6 / ORC_C * lnorm * dH;
+                dps[k][3] = -1e6 / ORC_C * lnorm * dL;
+                dph[k][4] = -v / dw * dH;
+                dps[k][4] = -v / dw * dL;
+                dph[k][5] = -dL;
+                dps[k][5] = dH;
+            }
+        }
+        double pmean = 0.5 * (ph[0] + ph[2]);
+        double pdiff = ph[1] - pmean;
+        double qdiff = ps[1] - 0.5 * (ps[0] + ps[2]);
+        double e0 = hEta * (ph[1] * st2 + pmean * ct2p1);
+        double e1 = pdiff * kQ, e2 = pdiff * kU, e3 = (ph[0] - ph[2]) * kV;
+        double r1 = qdiff * kQ, r2 = qdiff * kU, r3 = (ps[0] - ps[2]) * kV;
+
+        double A = 1 + e0;
+        double A2 = A * A;
+        double EE = e1 * e1 + e2 * e2 + e3 * e3;
+        double RR = r1 * r1 + r2 * r2 + r3 * r3;
+        double ER = e1 * r1 + e2 * r2 + e3 * r3;
+        double D = (A2 * (A2 - EE + RR) - ER * ER);
+        double sD = s1 / D;
+        double X1 = (e3 * r2 - e2 * r3);
+        double X2 = (e1 * r3 - e3 * r1);
+        double X3 = (e2 * r1 - e1 * r2);
+        oI[iw] = s0 + sD * (A * (A2 + RR));
+        oQ[iw] = -sD * (A2 * e1 + A * X1 + r1 * ER);
+        oU[iw] = -sD * (A2 * e2 + A * X2 + r2 * ER);
+        oV[iw] = -sD * (A2 * e3 + A * X3 + r3 * ER);
+
+        if (!jac) continue;
+
+        /* derivatives of eta_{I,Q,U,V} and rho_{Q,U,V} by parameter 0..6 */
+        double de[4][7], dr[4][7];
+        double pm = 0.5 * (dph[0][0] + dph[2][0]);
+        de[0][0] = hEta * (dph[1][0] * st2 + pm * ct2p1);
+        de[1][0] = (dph[1][0] - pm);
+        de[2][0] = de[1][0] * kU;
+        de[1][0] *= kQ;
+        de[3][0] = (dph[0][0] - dph[2][0]) * kV;
+        dr[1][0] = (dps[1][0] - 0.5 * (dps[0][0] + dps[2][0]));
+        dr[2][0] = dr[1][0] * kU;
+        dr[1][0] *= kQ;
+        dr[3][0] = (dps[0][0] - dps[2][0]) * kV;
+
+        de[0][1] = pdiff * kT;
+        de[1][1] = de[0][1] * c2p;
+        de[2][1] = de[0][1] * s2p;
+        de[3][1] = hEta * (ph[0] - ph[2]) * (-st);
+        dr[1][1] = qdiff * kT * c2p;
+        dr[2][1] = qdiff * kT * s2p;
+        dr[3][1] = hEta * (ps[0] - ps[2]) * (-st);
+
+        de[0][2] = 0;
+        de[1][2] = -2. * e2;
+        de[2][2] = 2. * e1;
+        de[3][2] = 0;
+        dr[1][2] = -2. * r2;
+        dr[2][2] = 2. * r1;
+        dr[3][2] = 0;
+
+        for (int p = 3; p < 6; p++) {
+            pm = 0.5 * (dph[0][p] + dph[2][p]);
+            de[0][p] = hEta * (dph[1][p] * st2 + pm * ct2p1);
+            de[1][p] = dph[1][p] - pm;
+            de[2][p] = de[1][p] * kU;
+            de[1][p] *= kQ;
+            de[3][p] = (dph[0][p] - dph[2][p]) * kV;
+            dr[1][p] = dps[1][p] - 0.5 * (dps[0][p] + dps[2][p]);
+            dr[2][p] = dr[1][p] * kU;
+            dr[1][p] *= kQ;
+            dr[3][p] = (dps[0][p] - dps[2][p]) * kV;
+        }
+        de[0][6] = e0 / eta0;
+        de[1][6] = e1 / eta0; dr[1][6] = r1 / eta0;
+        de[2][6] = e2 / eta0; dr[2][6] = r2 / eta0;
+        de[3][6] = e3 / eta0; dr[3][6] = r3 / eta0;
+
+        double *jI = jac, *jQ = jac + 9 * nl, *jU = jac + 18 * nl, *jV = jac + 27 * nl;
+        for (int p = 0; p < 7; p++) {
+            double dA = de[0][p];
+            double dEE = 2.0 * e1 * de[1][p] + 2.0 * e2 * de[2][p] + 2.0 * e3 * de[3][p];
+            double dRR = 2.0 * r1 * dr[1][p] + 2.0 * r2 * dr[2][p] + 2.0 * r3 * dr[3][p];
+            double dER = de[1][p] * r1 + e1 * dr[1][p] + de[2][p] * r2 + e2 * dr[2][p] + de[3][p] * r3
+                         + e3 * dr[3][p];
+            double dD = 2 * A * dA * (A2 - EE + RR) + A2 * (2 * A * dA - dEE + dRR) - 2.0 * ER * dER;
+            double sDD = sD / D;
+            jI[p * nl + iw] = (sDD * (D * (dA * (A2 + RR) + A * (2 * A * dA + dRR))
+                                      - dD * (A * (A2 + RR)))) * step[p];
+            jQ[p * nl + iw] = (-sDD * (D * (2 * A * dA * e1 + A2 * de[1][p] + dA * X1
+                                            + A * (de[3][p] * r2 + e3 * dr[2][p] - de[2][p] * r3
+                                                   - e2 * dr[3][p])
+                                            + dr[1][p] * ER + r1 * dER)
+                                       - dD * (A2 * e1 + A * (e3 * r2 - e2 * r3) + r1 * ER))) * step[p];
+            jU[p * nl + iw] = (-sDD * (D * (2 * A * dA * e2 + A2 * de[2][p] + dA * X2
+                                            + A * (de[1][p] * r3 + e1 * dr[3][p] - de[3][p] * r1
+                                                   - e3 * dr[1][p])
+                                            + dr[2][p] * ER + r2 * dER)
+                                       - dD * (A2 * e2 + A * (e1 * r3 - e3 * r1) + r2 * ER))) * step[p];
+            jV[p * nl + iw] = (-sDD * (D * (2 * A * dA * e3 + A2 * de[3][p] + dA * X3
+                                            + A * (de[2][p] * r1 + e2 * dr[1][p] - de[1][p] * r2
+                                                   - e1 * dr[2][p])
+                                            + dr[3][p] * ER + r3 * dER)
+                                       - dD * (A2 * e3 + A * (e2 * r1 - e1 * r2) + r3 * ER))) * step[p];
+        }
+        double T = (1 - beta) / D;
+        jI[7 * nl + iw] = (beta + T * (A * (A2 + RR))) * step[7];
+        jQ[7 * nl + iw] = (-T * (A2 * e1 + A * X1 + r1 * ER)) * step[7];
+        jU[7 * nl + iw] = (-T * (A2 * e2 + A * X2 + r2 * ER)) * step[7];
+        jV[7 * nl + iw] = (-T * (A2 * e3 + A * X3 + r3 * ER)) * step[7];
+        T = S / D;
+        jI[8 * nl + iw] = (S - T * (A * (A2 + RR))) * step[8];
+        jQ[8 * nl + iw] = (T * (A2 * e1 + A * X1 + r1 * ER)) * step[8];
+        jU[8 * nl + iw] = (T * (A2 * e2 + A * X2 + r2 * ER)) * step[8];
+        jV[8 * nl + iw] = (T * (A2 * e3 + A * X3 + r3 * ER)) * step[8];
+    }
+}
+
+void orc_synth_batch(const orc_ctx *c, const double *par, int64_t n, double *syn, double *jac)
+{
+    const int nl = c->nl;
+    for (int64_t p = 0; p < n; p++)
+        orc_synth(c, par + 9 * p, c->cfg.step, syn + 4 * nl * p, jac ? jac + 36 * nl * p : NULL, NULL);
+}
+
+/* ======================================================================== */
+/* bounds, noise, initial guess (Src/ME_SOLVER.c:52-69, 370-676)             */
+/* ======================================================================== */
+
+static inline double orc_clamp(double x, const double *b)
+{
+    return x < b[0] ? b[0] : (x > b[1] ? b[1] : x);
+}
+
+/* clamp B, fold theta into [0,pi], clamp 3..8, fold phi into [0,pi);
+ * Limits[1] and Limits[2] are never applied (Src/ME_SOLVER.c:370-406) */
+void orc_bounds_check(double *p, const double lim[9][2])
+{
+    const double twopi = 2.0 * ORC_PI;
+    p[0] = orc_clamp(p[0], lim[0]);
+    double t = fmod(p[1], twopi);
+    if (t < 0) t += twopi;
+    if (t > ORC_PI) t = twopi - t;
+    p[1] = t;
+    for (int k = 3; k < 9; k++) p[k] = orc_clamp(p[k], lim[k]);
+    double f = fmod(p[2], ORC_PI);
+    if (f < 0.0) f += ORC_PI;
+    p[2] = f;
+}
+
+/* I value at offset idx of the pixel block inside the contiguous batch
+ * buffer; reads before the first pixel are NaN (H7 at nl = 6) */
+static inline double orc_prof_at(const double *prof, int64_t pix, int nl, int idx)
+{
+    (void)nl;
+    if (idx < 0 && pix == 0) return NAN;
+    return prof[idx];
+}
+
+int orc_init_guess(const orc_ctx *c, const double *prof, int64_t pix, int64_t n, double *guess,
+                   double *limit7, double *step7, double *norm0, double *normp)
+{
+    (void)n;
+    const int nl = c->nl, ic = c->ic;
+    const double lam0 = c->cfg.lambda0;
+    const double *pI = prof, *pQ = prof + nl, *pU = prof + 2 * nl, *pV = prof + 3 * nl;
+    const double *lam = c->lambda;
+
+    double sI = 0., Imin = 1e10, sQ = 0., sU = 0., sV = 0.;
+    double Imax = 0, Lmax = 0., Vmax = 0., Vmin = 0.;
+    int iI = 0, iVmax = 0, iVmin = 0;
+    (void)iI;
+    double Blos = 0., Bpos = 0., thV, thL;
+    double Lp[512];
+
+    int iL = 0;
+    for (int iw = 0; iw < nl; iw++) {
+        if (Imax < pI[iw]) Imax = pI[iw];
+        if (Imin > pI[iw]) { iL = iw; Imin = pI[iw]; }
+        sI += pI[iw];
+        sQ += fabs(pQ[iw]);
+        sU += fabs(pU[iw]);
+        sV += fabs(pV[iw]);
+    }
+    for (int iw = c->i0; iw <= c->i1; iw++) {
+        Lp[iw] = sqrt(pQ[iw] * pQ[iw] + pU[iw] * pU[iw]);
+        if (Lmax < Lp[iw]) Lmax = Lp[iw];
+        if (Vmin > pV[iw]) {
+            iVmin = iw;
+            Vmin = pV[iw];
+        } else if (Vmax < pV[iw]) {
+            iVmax = iw;
+            Vmax = pV[iw];
+        }
+    }
+    double Imean = sI / nl;
+    if (Imean < c->cfg.icriteria) return 1;
+
+    double nz = 1e-2 * Imean;
+    *norm0 = nz * nz;
+    *normp = 1 + sQ * sQ / (sI * sI) * c->wsq[1] + sU * sU / (sI * sI) * c->wsq[2]
+             + sV * sV / (sI * sI) * c->wsq[3];
+
+    Lmax = 100 * Lmax / Imean;
+    Vmin = 100 * Vmin / Imean;
+    Vmax = 100 * Vmax / Imean;
+
+    if (lam0 > 15640 && lam0 < 15660) { thV = 1.5; thL = 2.5; }
+    else { thV = 5.0; thL = 5.0; }
+
+    double LV = 0.5 * (Vmax - Vmin);
+    double fac = Imin > 0.9 * Imax ? 2. : 1.;
+    if (LV > 0.1) {
+        Blos = 20 + c->cfg.vcoeffi * LV * fac;
+        if (iVmin < iVmax) Blos = -Blos;
+    }
+    Bpos = 100 + c->cfg.lcoeffi * Lmax * fac;
+    Bpos = Bpos < 2200 ? Bpos : 2200;
+
+    double dl;
+    guess[3] = 0;
+    if (LV > thV) {
+        dl = 0.5 * (lam[iVmin] + lam[iVmax]) - lam0;
+        guess[3] = dl / lam0 * ORC_C / 1e3;
+    } else {
+        int ok = 0;
+        if (Imin < 0.9 * Imax && iL >= c->ncut && iL < nl - c->ncut) {
+            /* H7: iL-2 may fall before the I block (previous pixel's V) */
+            if (pI[iL - 1] < orc_prof_at(prof, pix, nl, iL - 2) && pI[iL] < pI[iL - 1]
+                && pI[iL] < pI[iL + 1] && pI[iL + 1] < pI[iL + 2]) {
+                dl = lam[iL] - lam0;
+                ok = 1;
+            }
+        }
+        if (!ok) {
+            double num = 0, den = 0.;
+            int a = (ic > 7) ? (ic - 7) : 0;
+            int b = (ic + 7 < nl) ? (ic + 7) : (nl - 1);
+            for (int iw = a; iw <= b; iw++) {
+                const double d = (Imax - pI[iw]);
+                num += lam[iw] * d;
+                den += d;
+            }
+            dl = num / den - lam0;
+        }
+        guess[3] = dl / lam0 * ORC_C / 1e3;
+    }
+
+    guess[0] = sqrt(Blos * Blos + Bpos * Bpos);
+    guess[1] = acos(Blos / guess[0]);
+    if (guess[0] > 3000) guess[0] = 3000;
+
+    /* H8: Lp is absolute, the threshold is in percent of Imean */
+    double wsum = 0., phs = 0.;
+    for (int iw = c->i0; iw <= c->i1; iw++) {
+        if (Lp[iw] > thL) {
+            double t = 0.5 * atan2(pU[iw], pQ[iw]);
+            t = t > 0 ? t : ORC_PI + t;
+            phs += t * Lp[iw];
+            wsum += Lp[iw];
+        }
+    }
+    guess[2] = (wsum > 2. * thL) ? phs / wsum : ORC_PI / 2.;
+
+    if (lam0 > 15640 && lam0 < 15660) {
+        guess[4] = 60.; guess[5] = 0.5; guess[6] = 15; guess[8] = 0.7;
+    } else if (guess[0] > 1100) {
+        guess[4] = 20; guess[5] = 0.5; guess[6] = 20; guess[8] = 0.3;
+    } else {
+        guess[4] = 30; guess[5] = 0.5; guess[6] = 5; guess[8] = 0.15;
+    }
+    guess[7] = Imax;
+    limit7[0] = Imax * 0.6;
+    limit7[1] = Imax * 1.4;
+    *step7 = Imax * 0.1;
+    return 0;
+}
+
+/* ======================================================================== */
+/* Levenberg–Marquardt inversion (Src/LM_FIT.c:62-811)                       */
+/* ======================================================================== */
+
+typedef struct orc_work {
+    const orc_ctx *c;
+    int nl, np;
+    double step[9];
+    double lim[9][2];
+    double iv[4];
+    double normp;
+    double pbuf[3][9];
+    double *P, *Pb, *Pt;
+    double Pg[9];
+    double *syn, *fit, *fitb, *jac, *prof;
+    double H[81], V[81], W[9], g[9], sol[9];
+    double damp;
+    double lm_chisq;      /* H2: last trial chi2 of the previous LM_FIT */
+    double penalty, ratio;
+    double chisq, chisq_best;
+    int64_t ev[13];
+    int32_t iters, trials, svdfail;
+} orc_work;
+
+/* chi2 = sum (syn-prof)^2 ivnoise / (4 nl)  (Src/LM_FIT.c:89-120) */
+static double orc_loss(const orc_work *W)
+{
+    double acc = 0;
+    const int nl = W->nl;
+    for (int k = 0; k < 4 * nl; k++) {
+        double d = W->syn[k] - W->prof[k];
+        acc += d * d * W->iv[k / nl];
+    }
+    acc /= (4. * nl);
+    return acc;
+}
+
+/* normal matrix and gradient (Src/LM_FIT.c:240-350) */
+static void orc_hessian(orc_work *W)
+{
+    const int nl = W->nl, np = W->np;
+    const double *cfgfixed_dummy = NULL;
+    (void)cfgfixed_dummy;
+    int map[9], m = 0;
+    for (int p = 0; p < 9; p++)
+        if (!W->c->cfg.fixed[p]) map[m++] = p;
+    for (int a = 0; a < np; a++) {
+        int pa = map[a];
+        for (int b = a; b < np; b++) {
+            int pb = map[b];
+            double acc = 0;
+            for (int iw = 0; iw < nl; iw++) {
+                for (int s = 0; s < 4; s++) {
+                    const double *J = W->jac + 9 * nl * s;
+                    acc += W->iv[s] * J[pa * nl + iw] * J[pb * nl + iw];
+                }
+            }
+            W->H[b * np + a] = acc;
+            W->H[a * np + b] = acc;
+        }
+        double acc = 0;
+        for (int iw = 0; iw < nl; iw++) {
+            for (int s = 0; s < 4; s++) {
+                const double *J = W->jac + 9 * nl * s;
+                acc -= W->iv[s] * J[pa * nl + iw] * (W->syn[s * nl + iw] - W->prof[s * nl + iw]);
+            }
+        }
+        W->g[a] = acc;
+    }
+    W->ev[2]++;
+}
+
+/* Penalty = sum w (p - p0)^2; with the matrix flag add ratio*w to the
+ * diagonal and subtract ratio*w*(p-p0) from the gradient (Src/LM_FIT.c:354-396) */
+static void orc_regul(orc_work *W, const double *P, int matrix)
+{
+    const me_config *cf = &W->c->cfg;
+    int ip = -1;
+    W->penalty = 0.0;
+    for (int k = 0; k < 9; k++) {
+        if (cf->fixed[k]) continue;
+        ip++;
+        if (!cf->regul[k]) continue;
+        double r = P[k] - cf->regul_value[k];
+        W->penalty += r * r * cf->regul_weight[k];
+        if (matrix) {
+            double rj = 1. * r * cf->regul_weight[k];
+            double rh = 1. * 1. * cf->regul_weight[k];
+            W->g[ip] -= rj * W->ratio;
+            W->H[ip * W->np + ip] += rh * W->ratio;
+        }
+    }
+}
+
+/* truncated SVD solve of the matrix held in H, which SVD overwrites with U
+ * (H1); thresholds grow x5 until |step| <= 0.5 or clip  (Src/LM_FIT.c:400-469) */
+static void orc_svd_solve(orc_work *W)
+{
+    const int np = W->np;
+    const double maxstep = 0.5;
+    double thr = W->c->cfg.svd_threshold;
+    if (orc_svd(W->H, np, np, W->W, W->V)) W->svdfail = 1;
+    W->ev[3]++;
+    double wmax = W->W[0];
+    for (int i = 1; i < np; i++) wmax = wmax > W->W[i] ? wmax : W->W[i];
+    for (;;) {
+        double wt = wmax * thr;
+        for (int i = 0; i < np; i++)
+            if (W->W[i] < wt) W->W[i] = 0.;
+        orc_svbksb(W->H, W->V, W->W, np, np, W->g, W->sol);
+        W->ev[4]++;
+        int ok = 1;
+        for (int i = 0; i < np; i++)
+            if (W->sol[i] > maxstep || W->sol[i] < -maxstep) { ok = 0; break; }
+        if (ok) break;
+        if (thr > 2e-3) {
+            for (int i = 0; i < np; i++) {
+                if (W->sol[i] > maxstep) W->sol[i] = maxstep;
+                else if (W->sol[i] < -maxstep) W->sol[i] = -maxstep;
+            }
+            break;
+        }
+        thr *= 5.;
+    }
+}
+
+static void orc_synth_w(orc_work *W, const double *P, int deriv)
+{
+    orc_synth(W->c, P, W->step, W->syn, deriv ? W->jac : NULL, W->ev + 5);
+    W->ev[deriv ? 0 : 1]++;
+}
+
+/* Src/LM_FIT.c:473-527 (the damped copy of the matrix is never used: H1) */
+static double orc_trial(orc_work *W)
+{
+    orc_svd_solve(W);
+    for (int k = 0; k < 9; k++) W->Pt[k] = W->P[k];
+    int ip = 0;
+    for (int k = 0; k < 9; k++) {
+        if (W->c->cfg.fixed[k]) continue;
+        W->Pt[k] += W->sol[ip] * W->step[k];
+        ip++;
+    }
+    orc_bounds_check(W->Pt, (const double(*)[2])W->lim);
+    orc_synth_w(W, W->Pt, 0);
+    W->trials++;
+    W->lm_chisq = orc_loss(W);
+    if (W->c->regul_any) orc_regul(W, W->Pt, 0);
+    return W->lm_chisq;
+}
+
+#define ORC_SWAP(a, b) do { double *t_ = (a); (a) = (b); (b) = t_; } while (0)
+
+/* one LM run (Src/LM_FIT.c:531-661); returns the iteration count */
+static int orc_lm_fit(orc_work *W)
+{
+    const me_config *cf = &W->c->cfg;
+    const double limit = 0.1;
+    orc_synth_w(W, W->P, 1);
+    W->ratio = 1.0;
+    double chisq = orc_loss(W);
+    if (W->c->regul_any) {
+        orc_regul(W, W->P, 0);
+        if (W->penalty > limit * W->lm_chisq) W->ratio = limit * chisq / W->penalty;
+    }
+    int it;
+    int accepted = 0;
+    double cn = chisq;
+    double dp = W->damp;
+    for (it = 0; it < cf->num_iter; it++) {
+        accepted = 0;
+        orc_hessian(W);
+        if (W->c->regul_any) orc_regul(W, W->P, 1);
+        do {
+            cn = orc_trial(W);
+            if (cn < chisq) {
+                if (W->c->regul_any && W->penalty > limit * W->lm_chisq)
+                    W->ratio = limit * chisq / W->penalty;
+                accepted = 1;
+                ORC_SWAP(W->P, W->Pt);
+                ORC_SWAP(W->syn, W->fit);
+                orc_synth_w(W, W->P, 1);
+                W->damp /= cf->lambda_accept;
+                if (W->damp < cf->lambda_limits[0]) W->damp = cf->lambda_limits[0];
+            } else {
+                W->damp *= cf->lambda_reject;
+                if (W->damp > cf->lambda_limits[1]) break;
+            }
+        } while (!accepted);
+        W->damp = dp;
+        if (accepted) {
+            if (((chisq - cn) / chisq) < cf->convg_criteria) {
+                chisq = cn;
+                break;
+            }
+            chisq = cn;
+        } else {
+            break;
+        }
+    }
+    W->iters += it;
+    W->chisq = chisq / W->normp;
+    return it;
+}
+
+static void orc_sav(orc_work *W)
+{
+    ORC_SWAP(W->P, W->Pb);
+    ORC_SWAP(W->fit, W->fitb);
+    W->chisq_best = W->chisq;
+}
+
+/* Src/LM_FIT.c:157-201 with Philox4x32-10 Box–Muller draws for irun >= 4 */
+static void orc_jump(orc_work *W, int64_t pix, int irun)
+{
+    double *P = W->P, *B = W->Pb, *G = W->Pg;
+    if (irun < 4) {
+        for (int k = 0; k < 9; k++) P[k] = B[k];
+        P[2] = G[2] + ORC_PI / 4. * irun;
+    } else {
+        double n[8];
+        orc_gauss4(W->c->cfg.seed, pix, (uint32_t)irun, ORC_STREAM_JUMP + 0, n);
+        orc_gauss4(W->c->cfg.seed, pix, (uint32_t)irun, ORC_STREAM_JUMP + 1, n + 4);
+        P[0] = G[0] * (1 + 0.6 * n[0]);
+        P[1] = G[1] + ORC_PI / 8. * n[1];
+        P[2] = G[2] + ORC_PI / 4. * (irun + 0.3 * n[2]);
+        P[3] = G[3] + 5. * n[3];
+        P[4] = B[4] * (1 + 0.2 * n[4]);
+        P[5] = G[5] + 0.1 * n[5];
+        P[6] = G[6] + 5 * n[6];
+        P[7] = B[7];
+        P[8] = G[8] + 0.1 * n[7];
+    }
+    orc_bounds_check(P, (const double(*)[2])W->lim);
+}
+
+/* Src/LM_FIT.c:665-742; returns the run counter (0 = not inverted) */
+static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init, int32_t *runs)
+{
+    const me_config *cf = &W->c->cfg;
+    double lim7[2], step7, norm0;
+    *runs = 0;
+    if (orc_init_guess(W->c, W->prof, pix, n, W->Pg, lim7, &step7, &norm0, &W->normp)) {
+        W->chisq_best = -1.;
+        W->normp = 1.;
+        for (int k = 0; k < 9; k++) W->Pb[k] = 0;
+        return 0;
+    }
+    W->lim[7][0] = lim7[0];
+    W->lim[7][1] = lim7[1];
+    W->step[7] = step7;
+    if (init)
+        for (int k = 0; k < 9; k++) W->Pg[k] = init[k];
+    orc_bounds_check(W->Pg, (const double(*)[2])W->lim);
+    for (int k = 0; k < 9; k++) W->P[k] = W->Pg[k];
+
+    for (int s = 0; s < 4; s++) W->iv[s] = W->c->wsq[s] * (1. / norm0);
+
+    int irun;
+    for (irun = 0; irun < cf->num_run * 4; irun++) {
+        if (irun > 0) orc_jump(W, pix, irun);
+        orc_lm_fit(W);
+        (*runs)++;
+        if (W->chisq < cf->chisq_criteria) {
+            orc_sav(W);
+            if (irun >= 4 || irun == cf->num_run * 4) break;
+        } else if (irun == 0) {
+            orc_sav(W);
+        } else if (W->chisq < W->chisq_best) {
+            orc_sav(W);
+        }
+        if (irun >= cf->num_run * 2) {
+            if (W->chisq_best < cf->chisq_criteria * 2) break;
+        }
+    }
+    if (W->Pb[1] > ORC_PI) W->Pb[1] -= ORC_PI;
+    if (cf->hmi_ref) {
+        W->Pb[2] += 0.5 * ORC_PI;
+        if (W->Pb[2] > ORC_PI) W->Pb[2] -= ORC_PI;
+    }
+    return irun;
+}
+
+int orc_run_policy(int nruns, double criteria, const double *chisq_seq, int nseq, int *best_run)
+{
+    int irun, nexec = 0;
+    double best = 0;
+    *best_run = -1;
+    for (irun = 0; irun < nruns * 4; irun++) {
+        double c = irun < nseq ? chisq_seq[irun] : chisq_seq[nseq - 1];
+        nexec++;
+        if (c < criteria) {
+            best = c; *best_run = irun;
+            if (irun >= 4 || irun == nruns * 4) break;
+        } else if (irun == 0) {
+            best = c; *best_run = irun;
+        } else if (c < best) {
+            best = c; *best_run = irun;
+        }
+        if (irun >= nruns * 2) {
+            if (best < criteria * 2) break;
+        }
+    }
+    return nexec;
+}
+
+static void orc_work_init(orc_work *W, const orc_ctx *c, double *mem)
+{
+    memset(W, 0, sizeof(*W));
+    W->c = c;
+    W->nl = c->nl;
+    W->np = c->npar;
+    for (int k = 0; k < 9; k++) {
+        W->step[k] = c->cfg.step[k];
+        W->lim[k][0] = c->limits[k][0];
+        W->lim[k][1] = c->limits[k][1];
+    }
+    W->P = W->pbuf[0];
+    W->Pb = W->pbuf[1];
+    W->Pt = W->pbuf[2];
+    const int nl = c->nl;
+    W->syn = mem;
+    W->fit = mem + 4 * nl;
+    W->fitb = mem + 8 * nl;
+    W->jac = mem + 12 * nl;
+    W->damp = c->cfg.damp;
+    W->lm_chisq = 0.0;
+}
+
+/* one pixel of INVERSION_MULTI (Src/LM_FIT.c:774-808) */
+static void orc_pixel(orc_work *W, const double *prof, int64_t p, int64_t n, double *res,
+                      double *fit, const double *init, int h2_mode, orc_pixstats *ps)
+{
+    const int nl = W->nl;
+    W->prof = (double *)prof;
+    W->iters = W->trials = W->svdfail = 0;
+    double mean = 0.0;
+    for (int iw = 0; iw < nl; iw++) mean += prof[iw];
+    mean /= nl;
+    int valid = !(isnan(mean) || mean < W->c->cfg.icriteria);
+    int32_t runs = 0;
+    if (valid) {
+        if (h2_mode == ORC_H2_PER_PIXEL) W->lm_chisq = INFINITY;
+        if (!orc_inversion(W, p, n, init, &runs)) valid = 0;
+    }
+    if (valid) {
+        for (int k = 0; k < 9; k++) res[k] = W->Pb[k];
+        res[9] = W->chisq_best;
+        if (fit) memcpy(fit, W->fitb, sizeof(double) * 4 * nl);
+    } else {
+        memset(res, 0, sizeof(double) * 10);
+        if (fit) memset(fit, 0, sizeof(double) * 4 * nl);
+    }
+    if (ps) {
+        ps->iters = W->iters;
+        ps->runs = runs;
+        ps->trials = W->trials;
+        ps->status = (valid ? 0 : ME_PIX_INVALID) | (W->svdfail ? ME_PIX_SVD_FAIL : 0);
+    }
+}
+
+void orc_invert(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
+                const double *init, int h2_mode, int nthreads, orc_pixstats *pst, int64_t *ev)
+{
+    const int nl = c->nl;
+    int64_t evt[13] = {0};
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        orc_work W;
+        double *mem = (double *)malloc(sizeof(double) * (48 * nl + 8));
+        orc_work_init(&W, c, mem);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 20)
+#endif
+        for (int64_t p = 0; p < n; p++) {
+            orc_pixel(&W, prof + 4 * nl * p, p, n, res + 10 * p, fit ? fit + 4 * nl * p : NULL,
+                      init ? init + 9 * p : NULL, h2_mode, pst ? pst + p : NULL);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        for (int k = 0; k < 13; k++) evt[k] += W.ev[k];
+        free(mem);
+    }
+    if (ev)
+        for (int k = 0; k < 13; k++) ev[k] = evt[k];
+}
+
+/* ======================================================================== */
+/* synthetic maps                                                            */
+/* ======================================================================== */
+
+void orc_gen_map(const orc_ctx *c, uint64_t seed, const double *ranges, double ff_frac,
+                 double ff_bmax, double sigma, int64_t pix0, int64_t n, float *prof, double *truth)
+{
+    const int nl = c->nl;
+    double *syn = (double *)malloc(sizeof(double) * 4 * nl);
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t i = 0; i < n; i++) {
+        int64_t p = pix0 + i;
+        uint32_t w[12];
+        for (uint32_t b = 0; b < 3; b++) {
+            uint32_t ctr[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), b, ORC_STREAM_TRUTH};
+            orc_philox(ctr, key, w + 4 * b);
+        }
+        double t[9];
+        for (int k = 0; k < 9; k++) t[k] = ranges[2 * k] + (ranges[2 * k + 1] - ranges[2 * k]) * orc_u01(w[k]);
+        if (ff_frac > 0 && orc_u01(w[9]) < ff_frac) t[0] = ff_bmax * orc_u01(w[0]);
+        orc_synth(c, t, c->cfg.step, syn, NULL, NULL);
+        for (int b = 0; b < nl; b++) {
+            double g[4];
+            orc_gauss4(seed, p, (uint32_t)b, ORC_STREAM_NOISE, g);
+            for (int k = 0; k < 4; k++) {
+                int idx = 4 * b + k;
+                prof[4 * nl * i + idx] = (float)(syn[idx] + sigma * t[7] * g[k]);
+            }
+        }
+        if (truth)
+            for (int k = 0; k < 9; k++) truth[9 * i + k] = t[k];
+    }
+    free(syn);
+}

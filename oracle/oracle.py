@@ -1,0 +1,259 @@
+"""Python binding of the CPU FP64 oracle (liboracle.so) and of the reference
+harness (oracle/_ref/libref.so, present only where the reference sources were
+built).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
+checker) and bench.py's cpu_baseline leg.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent
+sys.path.insert(0, str(REPO / "milne-eddington_amd"))
+from meinv.abi import MeConfig  # noqa: E402  (data layout of include/meinv.h)
+
+ORC_H2_SEQUENTIAL, ORC_H2_PER_PIXEL = 0, 1
+FADD_BRANCHES = ["916", "hui6", "asym1", "asym2", "asym3", "asym4", "y0", "nu0", "hum4"]
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class PixStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("runs", C.c_int32), ("trials", C.c_int32),
+                ("status", C.c_int32)]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        p = HERE / "liboracle.so"
+        if not p.exists():
+            raise RuntimeError("oracle/liboracle.so missing: run make -C oracle")
+        L = C.CDLL(str(p))
+        L.orc_faddeeva_vec.argtypes = [C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _ip]
+        L.orc_svd.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_svd.restype = C.c_int
+        L.orc_svbksb.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_ctx_create.argtypes = [C.POINTER(MeConfig), _dp, C.c_int]
+        L.orc_ctx_create.restype = C.c_void_p
+        L.orc_ctx_destroy.argtypes = [C.c_void_p]
+        L.orc_ctx_info.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _dp]
+        L.orc_synth.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int64)]
+        L.orc_synth_batch.argtypes = [C.c_void_p, _dp, C.c_int64, _dp, _dp]
+        L.orc_bounds_check.argtypes = [_dp, _dp]
+        L.orc_init_guess.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+        L.orc_init_guess.restype = C.c_int
+        L.orc_invert.argtypes = [C.c_void_p, _dp, C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int,
+                                 C.POINTER(PixStats), C.POINTER(C.c_int64)]
+        L.orc_gen_map.argtypes = [C.c_void_p, C.c_uint64, _dp, C.c_double, C.c_double, C.c_double,
+                                  C.c_int64, C.c_int64, _fp, _dp]
+        L.orc_run_policy.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _ip]
+        L.orc_run_policy.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own Faddeeva/SVD built from its sources, or None."""
+    global _ref
+    if _ref is None:
+        p = HERE / "_ref" / "libref.so"
+        if not p.exists():
+            return None
+        L = C.CDLL(str(p))
+        L.ref_faddeeva_vec.argtypes = [C.c_int64, _dp, _dp, C.c_int, _dp, _dp]
+        L.ref_svd.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.ref_svbksb.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp]
+        _ref = L
+    return _ref
+
+
+# ---------------------------------------------------------------- Faddeeva
+def faddeeva(nu, y, nfigures=6):
+    nu = np.ascontiguousarray(nu, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(np.broadcast_to(y, nu.shape), dtype=np.float64).ravel()
+    H = np.empty_like(nu)
+    L = np.empty_like(nu)
+    br = np.empty(nu.shape, dtype=np.int32)
+    lib().orc_faddeeva_vec(nu.size, _ptr(nu, _dp), _ptr(y, _dp), nfigures, _ptr(H, _dp),
+                           _ptr(L, _dp), br.ctypes.data_as(_ip))
+    return H, L, br
+
+
+def ref_faddeeva(nu, y, nfigures=6):
+    R = ref_lib()
+    nu = np.ascontiguousarray(nu, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(np.broadcast_to(y, nu.shape), dtype=np.float64).ravel()
+    H = np.empty_like(nu)
+    L = np.empty_like(nu)
+    R.ref_faddeeva_vec(nu.size, _ptr(nu, _dp), _ptr(y, _dp), nfigures, _ptr(H, _dp), _ptr(L, _dp))
+    return H, L
+
+
+# ---------------------------------------------------------------- SVD
+def svd(a):
+    a = np.array(a, dtype=np.float64, order="C")
+    n = a.shape[1]
+    w = np.empty(n)
+    v = np.empty((n, n))
+    st = lib().orc_svd(_ptr(a, _dp), a.shape[0], n, _ptr(w, _dp), _ptr(v, _dp))
+    return a, w, v, st
+
+
+def ref_svd(a):
+    a = np.array(a, dtype=np.float64, order="C")
+    n = a.shape[0]
+    w = np.empty(n)
+    v = np.empty((n, n))
+    ref_lib().ref_svd(_ptr(a, _dp), n, _ptr(w, _dp), _ptr(v, _dp))
+    return a, w, v
+
+
+def svbksb(u, v, w, b):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(u.shape[1])
+    lib().orc_svbksb(_ptr(u, _dp), _ptr(v, _dp), _ptr(w, _dp), u.shape[0], u.shape[1], _ptr(b, _dp),
+                     _ptr(x, _dp))
+    return x
+
+
+def ref_svbksb(u, v, w, b):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(u.shape[0])
+    ref_lib().ref_svbksb(_ptr(u, _dp), _ptr(v, _dp), _ptr(w, _dp), u.shape[0], _ptr(b, _dp),
+                         _ptr(x, _dp))
+    return x
+
+
+# ---------------------------------------------------------------- Philox
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def bounds_check(p, limits):
+    p = np.array(p, dtype=np.float64)
+    lim = np.ascontiguousarray(limits, dtype=np.float64).reshape(9, 2)
+    lib().orc_bounds_check(_ptr(p, _dp), _ptr(lim, _dp))
+    return p
+
+
+def run_policy(nruns, criteria, seq):
+    seq = np.ascontiguousarray(seq, dtype=np.float64)
+    best = C.c_int(-1)
+    n = lib().orc_run_policy(nruns, criteria, _ptr(seq, _dp), seq.size, C.byref(best))
+    return n, best.value
+
+
+# ---------------------------------------------------------------- context
+class Oracle:
+    """One wavelength grid + configuration (INV_INIT derived constants)."""
+
+    def __init__(self, cfg: MeConfig, lam):
+        self.cfg = cfg
+        self.lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self.nl = self.lam.size
+        self.h = lib().orc_ctx_create(C.byref(cfg), _ptr(self.lam, _dp), self.nl)
+        if not self.h:
+            raise ValueError("orc_ctx_create failed")
+
+    def close(self):
+        if self.h:
+            lib().orc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        ic, i0, i1, nc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        v = np.zeros(2)
+        lib().orc_ctx_info(self.h, C.byref(ic), C.byref(i0), C.byref(i1), C.byref(nc), _ptr(v, _dp))
+        return dict(ic=ic.value, i0=i0.value, i1=i1.value, ncut=nc.value, vlos=tuple(v))
+
+    def synth(self, par, jac=False, step=None):
+        par = np.ascontiguousarray(par, dtype=np.float64).reshape(-1, 9)
+        n = par.shape[0]
+        syn = np.empty((n, 4, self.nl))
+        J = np.empty((n, 4, 9, self.nl)) if jac else None
+        if step is None:
+            lib().orc_synth_batch(self.h, _ptr(par, _dp), n, _ptr(syn, _dp), _ptr(J, _dp))
+        else:
+            st = np.ascontiguousarray(step, dtype=np.float64)
+            for i in range(n):
+                lib().orc_synth(self.h, _ptr(par[i], _dp), _ptr(st, _dp), _ptr(syn[i], _dp),
+                                _ptr(J[i], _dp) if jac else None, None)
+        return (syn, J) if jac else syn
+
+    def synth_counted(self, par):
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        syn = np.empty((4, self.nl))
+        cnt = np.zeros(9, dtype=np.int64)
+        st = np.array([self.cfg.step[i] for i in range(9)])
+        lib().orc_synth(self.h, _ptr(par, _dp), _ptr(st, _dp), _ptr(syn, _dp), None,
+                        cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        return syn, cnt
+
+    def init_guess(self, prof_batch, pix=0):
+        prof_batch = np.ascontiguousarray(prof_batch, dtype=np.float64).reshape(-1, 4, self.nl)
+        g = np.zeros(9)
+        l7 = np.zeros(2)
+        s7, n0, npn = C.c_double(), C.c_double(), C.c_double()
+        base = prof_batch.ctypes.data + pix * 4 * self.nl * 8
+        r = lib().orc_init_guess(self.h, C.cast(base, _dp), pix, prof_batch.shape[0], _ptr(g, _dp),
+                                 _ptr(l7, _dp), C.byref(s7), C.byref(n0), C.byref(npn))
+        return dict(fail=r, guess=g, limit7=l7, step7=s7.value, norm0=n0.value, normp=npn.value)
+
+    def invert(self, prof, init=None, fit=False, h2=ORC_H2_PER_PIXEL, nthreads=1):
+        prof = np.ascontiguousarray(prof, dtype=np.float64).reshape(-1, 4, self.nl)
+        n = prof.shape[0]
+        res = np.empty((n, 10))
+        F = np.empty((n, 4, self.nl)) if fit else None
+        ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64).reshape(n, 9)
+        ps = (PixStats * max(n, 1))()
+        ev = np.zeros(13, dtype=np.int64)
+        lib().orc_invert(self.h, _ptr(prof, _dp), n, _ptr(res, _dp), _ptr(F, _dp), _ptr(ini, _dp),
+                         h2, nthreads, ps, ev.ctypes.data_as(C.POINTER(C.c_int64)))
+        st = np.array([(ps[i].iters, ps[i].runs, ps[i].trials, ps[i].status) for i in range(n)],
+                      dtype=np.int32).reshape(n, 4)
+        out = dict(res=res, iters=st[:, 0], runs=st[:, 1], trials=st[:, 2], status=st[:, 3], events=ev)
+        if fit:
+            out["fit"] = F
+        return out
+
+    def gen_map(self, seed, ranges, n, ff_frac=0.0, ff_bmax=50.0, sigma=1e-3, pix0=0):
+        rg = np.ascontiguousarray(ranges, dtype=np.float64).reshape(9, 2)
+        prof = np.empty((n, 4, self.nl), dtype=np.float32)
+        truth = np.empty((n, 9))
+        lib().orc_gen_map(self.h, seed, _ptr(rg, _dp), ff_frac, ff_bmax, sigma, pix0, n,
+                          prof.ctypes.data_as(_fp), _ptr(truth, _dp))
+        return prof, truth
