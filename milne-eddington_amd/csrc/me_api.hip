@@ -1,0 +1,556 @@
+// me_api.hip — C ABI of libmeinv.so (include/meinv.h): context set-up
+// (INV_INIT's derived constants, Src/INV_INIT.c:121-205), device buffers,
+// launches of the synthesis, map and inversion kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/meinv.h"
+#include "me_invert.hpp"
+#include "me_mlp.hpp"
+
+using me::DevCfg;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct me_ctx {
+    me_config cfg;
+    int nl = 0, device = 0, ncu = 0;
+    DevCfg dc;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *d_lam = nullptr, *d_dlam = nullptr, *d_etab = nullptr;
+    float *d_dlamf = nullptr, *d_etabf = nullptr;
+    unsigned long long *d_queue = nullptr, *d_counters = nullptr;
+    DevBuf scratch, prof, res, fit, init, pxs, par, out, jac, fitpar;
+    me::MlpState mlp;
+};
+
+namespace {
+
+constexpr double kThr[5][5] = {{1.6e4, 160.0, 107.0, 28.5, 3.5},
+                               {1.5e5, 510.0, 110.0, 39., -0.1},
+                               {1.451e6, 1.6e3, 180.0, 110.0, -0.1},
+                               {1.5e7, 5.01e3, 380.0, 115.0, 114.0},
+                               {1.3e8, 1.6e4, 810.0, 195.0, 116.0}};
+
+me_status hip_status(hipError_t e)
+{
+    if (e == hipSuccess) return ME_OK;
+    if (e == hipErrorOutOfMemory) return ME_ENOMEM;
+    return ME_EDEVICE;
+}
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return hip_status(e_); \
+    } while (0)
+
+double ms_between(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return (double)ms;
+}
+
+// grid of the persistent inversion kernel: every resident slot of the chip,
+// but no more lanes than pixels
+template <typename K>
+void persistent_grid(const me_ctx *c, K kernel, int64_t n, int &block, int &grid)
+{
+    block = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
+    int per_cu = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+    if (per_cu < 1) per_cu = 1;
+    int waves_cu = per_cu * (block / 64);
+    if (c->cfg.waves_per_cu > 0) waves_cu = std::min(waves_cu, c->cfg.waves_per_cu);
+    int blocks_cu = std::max(1, waves_cu / (block / 64));
+    int64_t g = (int64_t)c->ncu * blocks_cu;
+    int64_t need = (n + block - 1) / block;
+    grid = (int)std::max<int64_t>(1, std::min(g, need));
+}
+
+template <typename T, typename TIn>
+me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res, double *d_fit,
+                        const double *d_init, int learned, int32_t *d_px, me_stats *st)
+{
+    auto kern = me::invert_kernel<T, TIn>;
+    int block, grid;
+    persistent_grid(c, kern, n, block, grid);
+    const int64_t nslots = (int64_t)grid * block;
+    HIPCHK(c->scratch.ensure(sizeof(double) * me::SL_N * nslots));
+    HIPCHK(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * me::CT_N, c->stream));
+    if (d_fit) HIPCHK(c->fitpar.ensure(sizeof(double) * 9 * n));
+    me::InvArgs A;
+    A.prof = d_prof;
+    A.n = n;
+    A.res = d_res;
+    A.fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
+    A.init = d_init;
+    A.learned = learned;
+    A.scratch = (double *)c->scratch.p;
+    A.nslots = nslots;
+    A.queue = c->d_queue;
+    A.counters = c->d_counters;
+    A.px_iters = d_px;
+    A.px_runs = d_px ? d_px + n : nullptr;
+    A.px_trials = d_px ? d_px + 2 * n : nullptr;
+    A.px_status = d_px ? d_px + 3 * n : nullptr;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, c->dc, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (d_fit) {
+        const int fb = 256, fg = (int)((n + fb - 1) / fb);
+        hipLaunchKernelGGL(me::fit_kernel<T>, dim3(fg), dim3(fb), 0, c->stream, c->dc, (const double *)c->fitpar.p, n,
+                           d_fit);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long ct[me::CT_N];
+    HIPCHK(hipMemcpyAsync(ct, c->d_counters, sizeof(ct), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st) {
+        st->kernel_ms = ms_between(c->ev0, c->ev1);
+        st->n_pixels = n;
+        st->n_valid = (int64_t)ct[me::CT_VALID];
+        st->n_converged = (int64_t)ct[me::CT_CONV];
+        st->n_runs = (int64_t)ct[me::CT_RUNS];
+        st->n_iters = (int64_t)ct[me::CT_ITERS];
+        st->n_jac = (int64_t)ct[me::CT_JAC];
+        st->n_trial = (int64_t)ct[me::CT_TRIAL];
+        st->n_hess = (int64_t)ct[me::CT_HESS];
+        st->n_svd = (int64_t)ct[me::CT_SVD];
+        st->n_bk = (int64_t)ct[me::CT_BK];
+        for (int b = 0; b < 8; b++) st->n_fadd[b] = (int64_t)ct[me::CT_F0 + b];
+        st->n_svd_fail = (int64_t)ct[me::CT_SVDFAIL];
+        st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
+    }
+    return ME_OK;
+}
+
+me_status invert_dispatch(me_ctx *c, const void *d_prof, int is_f64, int64_t n, double *d_res, double *d_fit,
+                          const double *d_init, int learned, int32_t *d_px, me_stats *st)
+{
+    if (c->cfg.precision == ME_FP32) {
+        return is_f64 ? launch_invert<float, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st)
+                      : launch_invert<float, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st);
+    }
+    return is_f64 ? launch_invert<double, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st)
+                  : launch_invert<double, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st);
+}
+
+me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double *res, double *fit,
+                      const double *init, me_stats *st)
+{
+    if (!c || n < 0 || (n > 0 && (!prof || !res))) return ME_EINVAL;
+    if (c->cfg.init_mode == ME_INIT_LEARNED) return ME_EINVAL;  // learned mode runs on device buffers
+    if (st) {
+        int32_t *keep[4] = {st->iters, st->runs, st->trials, st->status};
+        std::memset(st, 0, sizeof(*st));
+        st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
+    }
+    if (n == 0) return ME_OK;
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipEventRecord(t0, c->stream);
+    const int nl = c->nl;
+    HIPCHK(c->prof.ensure(esz * 4 * nl * n));
+    HIPCHK(c->res.ensure(sizeof(double) * 10 * n));
+    if (fit) HIPCHK(c->fit.ensure(sizeof(double) * 4 * nl * n));
+    if (init) HIPCHK(c->init.ensure(sizeof(double) * 9 * n));
+    const bool want_px = st && (st->iters || st->runs || st->trials || st->status);
+    if (want_px) HIPCHK(c->pxs.ensure(sizeof(int32_t) * 4 * n));
+    HIPCHK(hipMemcpyAsync(c->prof.p, prof, esz * 4 * nl * n, hipMemcpyHostToDevice, c->stream));
+    if (init) HIPCHK(hipMemcpyAsync(c->init.p, init, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+    me_status s = invert_dispatch(c, c->prof.p, esz == 8, n, (double *)c->res.p, fit ? (double *)c->fit.p : nullptr,
+                                  init ? (const double *)c->init.p : nullptr, 0,
+                                  want_px ? (int32_t *)c->pxs.p : nullptr, st);
+    if (s != ME_OK) return s;
+    HIPCHK(hipMemcpyAsync(res, c->res.p, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, c->stream));
+    if (fit) HIPCHK(hipMemcpyAsync(fit, c->fit.p, sizeof(double) * 4 * nl * n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<int32_t> px;
+    if (want_px) {
+        px.resize(4 * n);
+        HIPCHK(hipMemcpyAsync(px.data(), c->pxs.p, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    hipEventRecord(t1, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st) {
+        st->total_ms = ms_between(t0, t1);
+        int32_t *dst[4] = {st->iters, st->runs, st->trials, st->status};
+        for (int k = 0; k < 4; k++)
+            if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return ME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void me_config_defaults(me_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    const double pi = me::kPi;
+    c->num_iter = 60;
+    c->num_run = 4;
+    c->convg_criteria = 1e-3;
+    c->chisq_criteria = 1.0;
+    for (int i = 0; i < 4; i++) c->weights[i] = 1.0;
+    c->damp = 1e-3;
+    c->lambda_accept = 5.0;
+    c->lambda_reject = 5.0;
+    c->lambda_limits[0] = 1e-5;
+    c->lambda_limits[1] = 1e3;
+    const double b[9][2] = {{5, 6000}, {0, pi}, {0, pi}, {-20., 20.}, {10, 80}, {0.4, 0.8}, {2, 90}, {0, 1e6}, {0.1, 0.85}};
+    std::memcpy(c->bounds, b, sizeof(b));
+    c->regul[1] = 1;
+    c->regul_value[1] = 0.5 * pi;
+    c->regul_weight[1] = 1e-4;
+    c->regul_value[4] = 20.0; c->regul_weight[4] = 0.05;
+    c->regul_value[5] = 0.5;  c->regul_weight[5] = 80.0;
+    c->regul_value[6] = 2.0;  c->regul_weight[6] = 0.01;
+    c->vcoeffi = 100.0;
+    c->lcoeffi = 250.0;
+    c->icriteria = 100.0;
+    c->svd_threshold = 1e-6;
+    c->nfigures = 6;
+    const double st[9] = {1000., 2., 2., 2., 20., 0.3, 30., 1e5, 0.3};
+    std::memcpy(c->step, st, sizeof(st));
+    c->precision = ME_FP64;
+    c->init_mode = ME_INIT_HEURISTIC;
+    c->seed = 20261200;
+}
+
+me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_ctx **out)
+{
+    if (!cfg || !lambda || !out || nl < 2 || nl > 512) return ME_EINVAL;
+    *out = nullptr;
+    // the reference rejects these at parse time (Src/RINPUT.c:228-231)
+    if (cfg->lambda_accept < 1.5 || cfg->lambda_reject < 1.5) return ME_EINVAL;
+    if (cfg->nfigures < 1 || cfg->nfigures > 8) return ME_EINVAL;
+    if (!(cfg->svd_threshold > 0) || cfg->num_iter < 0 || cfg->num_run < 0) return ME_EINVAL;
+    // a zero damping or an infinite upper limit would make a rejected trial
+    // repeat forever (the reference loops there)
+    if (!(cfg->damp > 0) || !std::isfinite(cfg->lambda_limits[1]) || !std::isfinite(cfg->damp)) return ME_EINVAL;
+    if (cfg->block_size != 0 && (cfg->block_size < 64 || cfg->block_size > 256 || cfg->block_size % 64))
+        return ME_EINVAL;  // the kernel's LDS counters hold 256 lanes
+    if (cfg->precision != ME_FP64 && cfg->precision != ME_FP32) return ME_EINVAL;
+    for (int p = 0; p < 9; p++)
+        if (cfg->fixed[p]) return ME_EINVAL;  // the device path inverts all 9 parameters
+    for (int i = 0; i < nl; i++)
+        if (!std::isfinite(lambda[i])) return ME_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ME_EDEVICE;
+
+    me_ctx *c = new me_ctx();
+    c->cfg = *cfg;
+    c->nl = nl;
+    hipGetDevice(&c->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return ME_EDEVICE; }
+    c->ncu = prop.multiProcessorCount;
+
+    DevCfg &d = c->dc;
+    std::memset(&d, 0, sizeof(d));
+    d.nl = nl;
+    d.lambda0 = cfg->lambda0;
+    d.bshift = 4.6686e-10 * cfg->geff * cfg->lambda0 * cfg->lambda0;
+    for (int p = 0; p < 9; p++) {
+        d.lim[p][0] = cfg->bounds[p][0];
+        d.lim[p][1] = cfg->bounds[p][1];
+        d.step[p] = cfg->step[p];
+        d.regul[p] = cfg->regul[p];
+        d.rval[p] = cfg->regul_value[p];
+        d.rw[p] = cfg->regul_weight[p];
+        if (cfg->regul[p]) d.regul_any = 1;
+    }
+    d.ncut = nl / 5;
+    double t = (lambda[d.ncut] - cfg->lambda0) * me::kC / 1e3;
+    if (d.lim[3][0] < t) d.lim[3][0] = t;
+    t = (lambda[nl - 1 - d.ncut] - cfg->lambda0) * me::kC / 1e3;
+    if (d.lim[3][1] > t) d.lim[3][1] = t;
+    // line-centre index: the reference compares with a boolean (lambda0 >
+    // lambda[iw]), so ic is the last index below lambda0, 0 if none
+    d.ic = 0;
+    for (int iw = 0; iw < nl - 1; iw++) {
+        double below = cfg->lambda0 > lambda[iw];
+        double prod = lambda[iw + 1] * cfg->lambda0;
+        if (below > 0 && prod > 0) d.ic = below < prod ? iw : iw + 1;
+    }
+    int h1 = 2, h2 = 2;
+    if (d.ic - d.ncut >= 2) h1 = d.ic - d.ncut;
+    if (nl - 1 - d.ncut - d.ic >= 2) h2 = nl - 1 - d.ncut - d.ic;
+    int half = std::max(h1, h2);
+    d.i0 = d.ic - half;
+    if (d.i0 < 0) { d.i0 = 0; half = d.ic; }
+    d.i1 = d.ic + half;
+    if (d.i1 > nl - 1) { d.i1 = nl - 1; half = d.i1 - d.ic; d.i0 = d.ic - half; }
+    d.niter = cfg->num_iter;
+    d.nruns = cfg->num_run;
+    d.hmi_ref = cfg->hmi_ref;
+    d.nfig = cfg->nfigures;
+    d.icrit = cfg->icriteria;
+    d.crit = cfg->chisq_criteria;
+    d.convg = cfg->convg_criteria;
+    d.thr = cfg->svd_threshold;
+    d.damp0 = cfg->damp;
+    d.lacc = cfg->lambda_accept;
+    d.lrej = cfg->lambda_reject;
+    d.llim0 = cfg->lambda_limits[0];
+    d.llim1 = cfg->lambda_limits[1];
+    d.vco = cfg->vcoeffi;
+    d.lco = cfg->lcoeffi;
+    for (int s = 0; s < 4; s++) d.wsq[s] = cfg->weights[s] * cfg->weights[s];
+    const double rmin = std::pow(10., -cfg->nfigures);
+    d.fa = M_PI / std::sqrt(-std::log(0.5 * rmin));
+    d.faa = d.fa * d.fa;
+    d.flogr = -std::log(rmin);
+    d.fsqrtlogr = std::sqrt(d.flogr);
+    int ifig = std::min(4, cfg->nfigures > 4 ? cfg->nfigures - 4 : 0);
+    for (int k = 0; k < 5; k++) d.fthr[k] = kThr[ifig][k];
+    d.seed = cfg->seed;
+
+    double etab[50];
+    float etabf[50];
+    etab[0] = 1.;
+    for (int i = 1; i < 50; i++) etab[i] = std::exp(-d.faa * i * i);
+    for (int i = 0; i < 50; i++) etabf[i] = (float)etab[i];
+    std::vector<double> dl(nl);
+    std::vector<float> dlf(nl);
+    for (int i = 0; i < nl; i++) {
+        dl[i] = lambda[i] - cfg->lambda0;
+        dlf[i] = (float)dl[i];
+    }
+    hipError_t e = hipSuccess;
+    e = e ? e : hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    e = e ? e : hipEventCreate(&c->ev0);
+    e = e ? e : hipEventCreate(&c->ev1);
+    e = e ? e : hipMalloc(&c->d_lam, sizeof(double) * nl);
+    e = e ? e : hipMalloc(&c->d_dlam, sizeof(double) * nl);
+    e = e ? e : hipMalloc(&c->d_dlamf, sizeof(float) * nl);
+    e = e ? e : hipMalloc(&c->d_etab, sizeof(double) * 50);
+    e = e ? e : hipMalloc(&c->d_etabf, sizeof(float) * 50);
+    e = e ? e : hipMalloc(&c->d_queue, sizeof(unsigned long long) * 8);
+    e = e ? e : hipMalloc(&c->d_counters, sizeof(unsigned long long) * me::CT_N);
+    e = e ? e : hipMemcpy(c->d_lam, lambda, sizeof(double) * nl, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(c->d_dlam, dl.data(), sizeof(double) * nl, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(c->d_dlamf, dlf.data(), sizeof(float) * nl, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(c->d_etab, etab, sizeof(etab), hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(c->d_etabf, etabf, sizeof(etabf), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        me_ctx_destroy(c);
+        return hip_status(e);
+    }
+    d.lam = c->d_lam;
+    d.dlam = c->d_dlam;
+    d.dlamf = c->d_dlamf;
+    d.etab = c->d_etab;
+    d.etabf = c->d_etabf;
+    me::mlp_init(c->mlp, 4 * nl);
+    *out = c;
+    return ME_OK;
+}
+
+void me_ctx_destroy(me_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->d_lam);
+    hipFree(c->d_dlam);
+    hipFree(c->d_dlamf);
+    hipFree(c->d_etab);
+    hipFree(c->d_etabf);
+    hipFree(c->d_queue);
+    hipFree(c->d_counters);
+    DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar};
+    for (DevBuf *b : bufs) b->release();
+    me::mlp_free(c->mlp);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+me_status me_ctx_info(const me_ctx *c, int *ic, int *i0, int *i1, int *ncut, double vlos[2])
+{
+    if (!c) return ME_EINVAL;
+    if (ic) *ic = c->dc.ic;
+    if (i0) *i0 = c->dc.i0;
+    if (i1) *i1 = c->dc.i1;
+    if (ncut) *ncut = c->dc.ncut;
+    if (vlos) { vlos[0] = c->dc.lim[3][0]; vlos[1] = c->dc.lim[3][1]; }
+    return ME_OK;
+}
+
+me_status me_invert_batch(me_ctx *c, const float *prof, int64_t n, double *res, double *fit, const double *init,
+                          me_stats *st)
+{
+    return invert_host(c, prof, sizeof(float), n, res, fit, init, st);
+}
+
+me_status me_invert_batch_f64(me_ctx *c, const double *prof, int64_t n, double *res, double *fit,
+                              const double *init, me_stats *st)
+{
+    return invert_host(c, prof, sizeof(double), n, res, fit, init, st);
+}
+
+me_status me_invert_device(me_ctx *c, const void *d_prof, int prof_is_f64, int64_t n, double *d_res, double *d_fit,
+                           const double *d_init, me_stats *st)
+{
+    if (!c || n < 0 || (n > 0 && (!d_prof || !d_res))) return ME_EINVAL;
+    const int learned = c->cfg.init_mode == ME_INIT_LEARNED;
+    if (st) {
+        int32_t *keep[4] = {st->iters, st->runs, st->trials, st->status};
+        std::memset(st, 0, sizeof(*st));
+        st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
+    }
+    if (n == 0) return ME_OK;
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipEventRecord(t0, c->stream);
+    const double *dinit = d_init;
+    if (learned) {
+        if (!c->mlp.trained) return ME_ENOTREADY;
+        if (!dinit) {
+            HIPCHK(c->init.ensure(sizeof(double) * 9 * n));
+            me_status s = me::mlp_predict(c->mlp, c->dc, c->stream, (const float *)d_prof, prof_is_f64, n,
+                                          (double *)c->init.p);
+            if (s != ME_OK) return s;
+            dinit = (const double *)c->init.p;
+        }
+    }
+    const bool want_px = st && (st->iters || st->runs || st->trials || st->status);
+    if (want_px) HIPCHK(c->pxs.ensure(sizeof(int32_t) * 4 * n));
+    me_status s = invert_dispatch(c, d_prof, prof_is_f64, n, d_res, d_fit, dinit, learned,
+                                  want_px ? (int32_t *)c->pxs.p : nullptr, st);
+    if (s != ME_OK) return s;
+    if (want_px) {
+        std::vector<int32_t> px(4 * n);
+        HIPCHK(hipMemcpyAsync(px.data(), c->pxs.p, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int32_t *dst[4] = {st->iters, st->runs, st->trials, st->status};
+        for (int k = 0; k < 4; k++)
+            if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
+    }
+    hipEventRecord(t1, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st) st->total_ms = ms_between(t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return ME_OK;
+}
+
+me_status me_synth_batch(me_ctx *c, const double *par, int64_t n, double *stokes, double *jac)
+{
+    if (!c || n < 0 || (n > 0 && (!par || !stokes))) return ME_EINVAL;
+    if (n == 0) return ME_OK;
+    const int nl = c->nl;
+    HIPCHK(c->par.ensure(sizeof(double) * 9 * n));
+    HIPCHK(c->out.ensure(sizeof(double) * 4 * nl * n));
+    if (jac) HIPCHK(c->jac.ensure(sizeof(double) * 36 * nl * n));
+    HIPCHK(hipMemcpyAsync(c->par.p, par, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+    const int block = 256;
+    const int grid = (int)((n + block - 1) / block);
+    if (c->cfg.precision == ME_FP32)
+        hipLaunchKernelGGL(me::synth_kernel<float>, dim3(grid), dim3(block), 0, c->stream, c->dc,
+                           (const double *)c->par.p, n, (double *)c->out.p, jac ? (double *)c->jac.p : nullptr);
+    else
+        hipLaunchKernelGGL(me::synth_kernel<double>, dim3(grid), dim3(block), 0, c->stream, c->dc,
+                           (const double *)c->par.p, n, (double *)c->out.p, jac ? (double *)c->jac.p : nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(stokes, c->out.p, sizeof(double) * 4 * nl * n, hipMemcpyDeviceToHost, c->stream));
+    if (jac) HIPCHK(hipMemcpyAsync(jac, c->jac.p, sizeof(double) * 36 * nl * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ME_OK;
+}
+
+me_status me_gen_map_device(me_ctx *c, uint64_t seed, const double *ranges, double ff_frac, double ff_bmax,
+                            double sigma, int64_t n, float *d_prof, double *d_truth)
+{
+    if (!c || !ranges || n < 0 || (n > 0 && !d_prof)) return ME_EINVAL;
+    if (n == 0) return ME_OK;
+    double *d_rg = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_rg, sizeof(double) * 18, c->stream));
+    HIPCHK(hipMemcpyAsync(d_rg, ranges, sizeof(double) * 18, hipMemcpyHostToDevice, c->stream));
+    const int block = 256;
+    const int grid = (int)((n + block - 1) / block);
+    hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, c->dc, seed, (const double *)d_rg,
+                       ff_frac, ff_bmax, sigma, n, d_prof, d_truth);
+    hipError_t e = hipGetLastError();
+    hipFreeAsync(d_rg, c->stream);
+    HIPCHK(e);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ME_OK;
+}
+
+me_status me_mlp_train(me_ctx *c, uint64_t seed, const double *ranges, double ff_frac, double ff_bmax, double sigma,
+                       int64_t n_train, int batch, int epochs, double lr, double *final_loss)
+{
+    if (!c || !ranges || n_train <= 0 || batch <= 0 || epochs <= 0) return ME_EINVAL;
+    return me::mlp_train(c->mlp, c->dc, c->stream, seed, ranges, ff_frac, ff_bmax, sigma, n_train, batch, epochs, lr,
+                         final_loss);
+}
+
+me_status me_mlp_predict_device(me_ctx *c, const float *d_prof, int64_t n, double *d_init)
+{
+    if (!c || n < 0 || (n > 0 && (!d_prof || !d_init))) return ME_EINVAL;
+    if (!c->mlp.trained) return ME_ENOTREADY;
+    if (n == 0) return ME_OK;
+    return me::mlp_predict(c->mlp, c->dc, c->stream, d_prof, 0, n, d_init);
+}
+
+int64_t me_mlp_num_params(const me_ctx *c) { return c ? me::mlp_num_params(c->mlp) : 0; }
+
+me_status me_mlp_get_params(const me_ctx *c, float *params)
+{
+    if (!c || !params) return ME_EINVAL;
+    return me::mlp_get_params(const_cast<me::MlpState &>(c->mlp), c->stream, params);
+}
+
+me_status me_mlp_set_params(me_ctx *c, const float *params)
+{
+    if (!c || !params) return ME_EINVAL;
+    return me::mlp_set_params(c->mlp, c->stream, params);
+}
+
+me_status me_mlp_grad(me_ctx *c, const float *x, const float *y, int batch, float *grad, double *loss)
+{
+    if (!c || !x || !y || !grad || batch <= 0) return ME_EINVAL;
+    return me::mlp_grad_host(c->mlp, c->stream, x, y, batch, grad, loss);
+}
+
+const char *me_version(void) { return "libmeinv gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,691 @@
+// me_invert.hpp — persistent work-queue LM inversion kernel.
+//
+// Each lane owns one pixel at a time and runs the reference's per-pixel
+// procedure (Src/LM_FIT.c:665-811: INVERSION, LM_FIT, Trial_Synthesis,
+// SVD_solve_mem, Random_Jump) as a state machine; when its pixel is finished
+// it takes the next one from a device work counter (wave-aggregated atomic),
+// so uneven iteration and restart counts never idle a wavefront until the
+// queue is empty.  Each pass of the main loop does, per lane, at most one
+// Jacobian synthesis + normal-matrix build and exactly one trial step.
+#pragma once
+
+#include "me_device.hpp"
+
+namespace me {
+
+enum { PH_ACQ = 0, PH_JAC_START = 1, PH_JAC = 2, PH_TRIAL = 3, PH_DONE = 4 };
+
+// device counters (aggregates of me_stats)
+enum {
+    CT_JAC = 0, CT_TRIAL, CT_HESS, CT_SVD, CT_BK, CT_F0, CT_F1, CT_F2, CT_F3, CT_F4, CT_F5, CT_F6, CT_F7,
+    CT_RUNS, CT_ITERS, CT_VALID, CT_CONV, CT_SVDFAIL, CT_FALLBACK, CT_N
+};
+
+// per-lane slots in the scratch buffer (structure of arrays)
+// SL_MAT holds two 81-entry matrices: the trial's input A and its left
+// factor U alternate between them (mbuf selects the input)
+enum { SL_BEST = 0, SL_GUESS = 9, SL_LEARN = 18, SL_CHI_BEST = 27, SL_CHI_LEARN = 28, SL_G = 29, SL_MAT = 40,
+       SL_N = 40 + 162 };
+
+struct InvArgs {
+    const void *prof;
+    int64_t n;
+    double *res;
+    double *fitpar;  // [n][9] parameters of the best fit before HMI_REF (fit profiles are synthesised from them)
+    const double *init;
+    int learned;
+    double *scratch;
+    int64_t nslots;
+    unsigned long long *queue;
+    unsigned long long *counters;
+    int32_t *px_iters, *px_runs, *px_trials, *px_status;
+};
+
+template <typename TIn> __device__ __forceinline__ double ld(const void *p, int64_t i)
+{
+    return (double)((const TIn *)p)[i];
+}
+
+// Init_Guess of Src/ME_SOLVER.c:445-676 on the lane's pixel (FP64).
+template <typename TIn>
+__device__ void init_guess(const DevCfg &C, const void *prof, int64_t pix, double g[9], double &l7lo, double &l7hi,
+                           double &step7, double &norm0, double &normp)
+{
+    const int nl = C.nl;
+    const int64_t base = pix * 4 * nl;
+    double sI = 0., Imin = 1e10, sQ = 0., sU = 0., sV = 0.;
+    double Imax = 0, Lmax = 0., Vmax = 0., Vmin = 0.;
+    int iVmax = 0, iVmin = 0, iL = 0;
+    for (int iw = 0; iw < nl; iw++) {
+        double I = ld<TIn>(prof, base + iw);
+        if (Imax < I) Imax = I;
+        if (Imin > I) { iL = iw; Imin = I; }
+        sI += I;
+        sQ += fabs(ld<TIn>(prof, base + nl + iw));
+        sU += fabs(ld<TIn>(prof, base + 2 * nl + iw));
+        sV += fabs(ld<TIn>(prof, base + 3 * nl + iw));
+    }
+    for (int iw = C.i0; iw <= C.i1; iw++) {
+        double q = ld<TIn>(prof, base + nl + iw), u = ld<TIn>(prof, base + 2 * nl + iw);
+        double vv = ld<TIn>(prof, base + 3 * nl + iw);
+        double lp = sqrt(q * q + u * u);
+        if (Lmax < lp) Lmax = lp;
+        if (Vmin > vv) { iVmin = iw; Vmin = vv; }
+        else if (Vmax < vv) { iVmax = iw; Vmax = vv; }
+    }
+    const double Imean = sI / nl;
+    double nz = 1e-2 * Imean;
+    norm0 = nz * nz;
+    normp = 1 + sQ * sQ / (sI * sI) * C.wsq[1] + sU * sU / (sI * sI) * C.wsq[2] + sV * sV / (sI * sI) * C.wsq[3];
+    Lmax = 100 * Lmax / Imean;
+    Vmin = 100 * Vmin / Imean;
+    Vmax = 100 * Vmax / Imean;
+    const double lam0 = C.lambda0;
+    const bool niris = lam0 > 15640 && lam0 < 15660;
+    const double thV = niris ? 1.5 : 5.0, thL = niris ? 2.5 : 5.0;
+    const double LV = 0.5 * (Vmax - Vmin);
+    const double fac = Imin > 0.9 * Imax ? 2. : 1.;
+    double Blos = 0.;
+    if (LV > 0.1) {
+        Blos = 20 + C.vco * LV * fac;
+        if (iVmin < iVmax) Blos = -Blos;
+    }
+    double Bpos = 100 + C.lco * Lmax * fac;
+    Bpos = Bpos < 2200 ? Bpos : 2200;
+    double dl;
+    if (LV > thV) {
+        dl = 0.5 * (C.lam[iVmin] + C.lam[iVmax]) - lam0;
+    } else {
+        bool ok = false;
+        if (Imin < 0.9 * Imax && iL >= C.ncut && iL < nl - C.ncut) {
+            // at small nl the reference reads I[iL-2] before the pixel's I
+            // block (the previous pixel's V); before the first pixel: NaN
+            const int64_t im2 = base + iL - 2;
+            double Im2 = im2 < 0 ? (double)NAN : ld<TIn>(prof, im2);
+            double Im1 = ld<TIn>(prof, base + iL - 1), I0 = ld<TIn>(prof, base + iL);
+            double Ip1 = ld<TIn>(prof, base + iL + 1), Ip2 = ld<TIn>(prof, base + iL + 2);
+            if (Im1 < Im2 && I0 < Im1 && I0 < Ip1 && Ip1 < Ip2) {
+                dl = C.lam[iL] - lam0;
+                ok = true;
+            }
+        }
+        if (!ok) {
+            double num = 0, den = 0.;
+            int a = (C.ic > 7) ? (C.ic - 7) : 0;
+            int b = (C.ic + 7 < nl) ? (C.ic + 7) : (nl - 1);
+            for (int iw = a; iw <= b; iw++) {
+                const double d = (Imax - ld<TIn>(prof, base + iw));
+                num += C.lam[iw] * d;
+                den += d;
+            }
+            dl = num / den - lam0;
+        }
+    }
+    g[3] = dl / lam0 * kC / 1e3;
+    g[0] = sqrt(Blos * Blos + Bpos * Bpos);
+    g[1] = acos(Blos / g[0]);
+    if (g[0] > 3000) g[0] = 3000;
+    double wsum = 0., phs = 0.;
+    for (int iw = C.i0; iw <= C.i1; iw++) {
+        double q = ld<TIn>(prof, base + nl + iw), u = ld<TIn>(prof, base + 2 * nl + iw);
+        double lp = sqrt(q * q + u * u);
+        if (lp > thL) {
+            double t = 0.5 * atan2(u, q);
+            t = t > 0 ? t : kPi + t;
+            phs += t * lp;
+            wsum += lp;
+        }
+    }
+    g[2] = (wsum > 2. * thL) ? phs / wsum : kPi / 2.;
+    if (niris) { g[4] = 60.; g[5] = 0.5; g[6] = 15; g[8] = 0.7; }
+    else if (g[0] > 1100) { g[4] = 20; g[5] = 0.5; g[6] = 20; g[8] = 0.3; }
+    else { g[4] = 30; g[5] = 0.5; g[6] = 5; g[8] = 0.15; }
+    g[7] = Imax;
+    l7lo = Imax * 0.6;
+    l7hi = Imax * 1.4;
+    step7 = Imax * 0.1;
+}
+
+__device__ __forceinline__ double penalty_of(const DevCfg &C, const double p[9])
+{
+    double pen = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        if (C.regul[k]) {
+            double r = p[k] - C.rval[k];
+            pen += r * r * C.rw[k];
+        }
+    }
+    return pen;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// one atomic per wave hands consecutive pixel indices to the lanes that ask
+__device__ __forceinline__ int64_t wave_acquire(unsigned long long *q, bool need)
+{
+    const uint64_t mask = __ballot(need);
+    const int cnt = __popcll(mask);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (__lane_id() == leader) base = atomicAdd(q, (unsigned long long)cnt);
+    base = __shfl(base, leader);
+    return (int64_t)(base + __popcll(mask & lanemask_lt()));
+}
+
+// Jacobian rows -> normal matrix and gradient, same accumulation order as
+// Hessian_Compute (Src/LM_FIT.c:291-314): wavelength outer, Stokes inner
+template <typename T> struct HessSink {
+    T *hacc, *gacc;
+    const T *d;     // residual syn - prof of the 4 Stokes at this wavelength
+    const T *iv;    // 4 inverse noise variances
+    __device__ __forceinline__ void operator()(int s, const T *row) const
+    {
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < 9; a++) {
+            const T ja = iv[s] * row[a];
+#pragma unroll
+            for (int b = a; b < 9; b++) hacc[idx++] += ja * row[b];
+            gacc[a] -= ja * d[s];
+        }
+    }
+};
+
+template <typename T, typename TIn>
+__global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
+{
+    const int nl = C.nl;
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double *scr = A.scratch + slot;
+    const int64_t NS = A.nslots;
+#define SCR(k) scr[(int64_t)(k) * NS]
+    // event counters live in LDS, one column per lane
+    __shared__ uint32_t lct[CT_N][256];
+    uint32_t *myct = &lct[0][threadIdx.x];
+#define CT(k) myct[(k) * 256]
+#pragma unroll
+    for (int k = 0; k < CT_N; k++) CT(k) = 0;
+
+    int phase = PH_ACQ;
+    int64_t pix = -1;
+    int irun = 0, it = 0, lphase = 0, mbuf = 0;
+    double par[9];
+    double chisq = 0, damp = 0, ratio = 1, lmchisq = 0, normp = 1, ivn = 1, step7 = 0, l7lo = 0, l7hi = 0;
+    int32_t px_iters = 0, px_runs = 0, px_trials = 0, px_status = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) par[k] = 0;
+
+    for (;;) {
+        // ---------------------------------------------------- acquire pixels
+        for (;;) {
+            const bool need = (phase == PH_ACQ);
+            if (__ballot(need) == 0) break;
+            const int64_t p = wave_acquire(A.queue, need);
+            if (!need) continue;
+            if (p >= A.n) { phase = PH_DONE; continue; }
+            pix = p;
+            const int64_t base = pix * 4 * nl;
+            double mean = 0.0;
+            for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(A.prof, base + iw);
+            mean /= nl;
+            const bool valid = !(isnan(mean) || mean < C.icrit) && (C.nruns > 0 || A.learned);
+            if (!valid) {
+#pragma unroll
+                for (int k = 0; k < 10; k++) A.res[pix * 10 + k] = 0.0;
+                if (A.fitpar) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) A.fitpar[pix * 9 + k] = 0.0;
+                }
+                if (A.px_iters) {
+                    A.px_iters[pix] = 0;
+                    A.px_runs[pix] = 0;
+                    A.px_trials[pix] = 0;
+                    A.px_status[pix] = 1;
+                }
+                continue;  // stays PH_ACQ
+            }
+            double gs[9], norm0;
+            init_guess<TIn>(C, A.prof, pix, gs, l7lo, l7hi, step7, norm0, normp);
+            if (A.init && !A.learned) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) gs[k] = A.init[pix * 9 + k];
+            }
+            bounds_check(C, gs, l7lo, l7hi);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                SCR(SL_GUESS + k) = gs[k];
+                par[k] = gs[k];
+            }
+            if (A.learned) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) par[k] = A.init[pix * 9 + k];
+                bounds_check(C, par, l7lo, l7hi);
+                lphase = 1;
+            } else {
+                lphase = 0;
+            }
+            ivn = 1. / norm0;
+            irun = 0;
+            lmchisq = INFINITY;
+            px_iters = px_runs = px_trials = px_status = 0;
+            phase = PH_JAC_START;
+        }
+        if (__ballot(phase != PH_DONE) == 0) break;
+
+        bool ended = false;
+        const int64_t base = pix * 4 * nl;
+        const T ivT[4] = {T(C.wsq[0] * ivn), T(C.wsq[1] * ivn), T(C.wsq[2] * ivn), T(C.wsq[3] * ivn)};
+
+        // ---------------------------------------------- Jacobian + normal matrix
+        if (phase == PH_JAC_START || phase == PH_JAC) {
+            PixK<T> K;
+            pix_consts<T>(C, par, K);
+            T hacc[45], gacc[9];
+#pragma unroll
+            for (int k = 0; k < 45; k++) hacc[k] = 0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) gacc[k] = 0;
+            double chi = 0.0;
+            FCount fc;
+            for (int iw = 0; iw < nl; iw++) {
+                T o[4], d[4];
+                // the sink reads d, which is filled before the first row arrives
+                HessSink<T> sink{hacc, gacc, d, ivT};
+                me_point<T, true>(C, K, T(step7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+                    if (s == 0) {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            d[q] = o[q] - T(ld<TIn>(A.prof, base + q * nl + iw));
+                            chi += (double)d[q] * (double)d[q] * (C.wsq[q] * ivn);
+                        }
+                    }
+                    sink(s, row);
+                }, &fc);
+            }
+            chi /= (4. * nl);
+#pragma unroll
+            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((fc.lo >> (16 * b)) & 0xFFFF);
+#pragma unroll
+            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((fc.hi >> (16 * b)) & 0xFFFF);
+            CT(CT_JAC) += 1;
+            CT(CT_HESS) += 1;
+            if (phase == PH_JAC_START) {
+                chisq = chi;
+                ratio = 1.0;
+                it = 0;
+                if (C.regul_any) {
+                    const double pen = penalty_of(C, par);
+                    if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
+                }
+            }
+            // normal matrix (+ regularisation, Src/LM_FIT.c:373-393) into the
+            // input slot of the trial, gradient into SL_G
+            double gg[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) gg[k] = (double)gacc[k];
+            double diag[9];
+            {
+                int idx = 0;
+#pragma unroll
+                for (int a = 0; a < 9; a++) {
+#pragma unroll
+                    for (int b = a; b < 9; b++) {
+                        if (b == a) diag[a] = (double)hacc[idx];
+                        else {
+                            SCR(SL_MAT + 81 * mbuf + a * 9 + b) = (double)hacc[idx];
+                            SCR(SL_MAT + 81 * mbuf + b * 9 + a) = (double)hacc[idx];
+                        }
+                        idx++;
+                    }
+                }
+            }
+            if (C.regul_any) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    if (C.regul[k]) {
+                        const double r = par[k] - C.rval[k];
+                        gg[k] -= (r * C.rw[k]) * ratio;
+                        diag[k] += C.rw[k] * ratio;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                SCR(SL_MAT + 81 * mbuf + k * 10) = diag[k];
+                SCR(SL_G + k) = gg[k];
+            }
+            damp = C.damp0;
+            if (phase == PH_JAC_START && C.niter <= 0) ended = true;
+            else phase = PH_TRIAL;
+        }
+
+        // ------------------------------------------------------------ trial
+        if (phase == PH_TRIAL && !ended) {
+            // SVD of the input matrix (the normal matrix, or U of the previous
+            // rejected trial: H1); U goes to the other slot and becomes the
+            // next trial's input
+            double M[81], w[9];
+            double *in = scr + (int64_t)(SL_MAT + 81 * mbuf) * NS;
+            double *uo = scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS;
+#pragma unroll
+            for (int k = 0; k < 81; k++) M[k] = in[(int64_t)k * NS];
+            const int sf = svd9(M, w);
+            CT(CT_SVD) += 1;
+            if (sf) { px_status |= 2; CT(CT_SVDFAIL) += 1; }
+            double ug[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int i = 0; i < 9; i++) sacc += M[i * 9 + j] * SCR(SL_G + i);
+                ug[j] = sacc;
+            }
+#pragma unroll
+            for (int k = 0; k < 81; k++) uo[(int64_t)k * NS] = M[k];
+            double wmax = w[0];
+#pragma unroll
+            for (int i = 1; i < 9; i++) wmax = wmax > w[i] ? wmax : w[i];
+            // truncated solve (Src/LM_FIT.c:423-466): x = V W^+ U^T g with
+            // V W^+ t = A^T U W^-1 W^+ t; thresholds x5 until |x| <= 0.5
+            double thr = C.thr, sol[9];
+            for (int pass = 0; pass < 64; pass++) {
+                const double wt = wmax * thr;
+                double y[9];
+#pragma unroll
+                for (int j = 0; j < 9; j++) {
+                    if (w[j] < wt) w[j] = 0.;
+                    y[j] = w[j] > 1e-50 ? (ug[j] / w[j]) / w[j] : 0.0;
+                }
+                double uy[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 9; j++) sacc += M[k * 9 + j] * y[j];
+                    uy[k] = sacc;
+                }
+#pragma unroll
+                for (int i = 0; i < 9; i++) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 9; k++) sacc += in[(int64_t)(k * 9 + i) * NS] * uy[k];
+                    sol[i] = sacc;
+                }
+                CT(CT_BK) += 1;
+                bool ok = true;
+#pragma unroll
+                for (int i = 0; i < 9; i++) ok = ok && !(sol[i] > 0.5 || sol[i] < -0.5);
+                if (ok) break;
+                if (thr > 2e-3) {
+#pragma unroll
+                    for (int i = 0; i < 9; i++) sol[i] = sol[i] > 0.5 ? 0.5 : (sol[i] < -0.5 ? -0.5 : sol[i]);
+                    break;
+                }
+                thr *= 5.;
+            }
+            mbuf ^= 1;
+            double pt[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) pt[k] = par[k] + sol[k] * (k == 7 ? step7 : C.step[k]);
+            bounds_check(C, pt, l7lo, l7hi);
+
+            PixK<T> K;
+            pix_consts<T>(C, pt, K);
+            double chi = 0.0;
+            FCount fc;
+            for (int iw = 0; iw < nl; iw++) {
+                T o[4];
+                me_point<T, false>(C, K, T(step7), iw, o, NoSink{}, &fc);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const T d = o[q] - T(ld<TIn>(A.prof, base + q * nl + iw));
+                    chi += (double)d * (double)d * (C.wsq[q] * ivn);
+                }
+            }
+            chi /= (4. * nl);
+#pragma unroll
+            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((fc.lo >> (16 * b)) & 0xFFFF);
+#pragma unroll
+            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((fc.hi >> (16 * b)) & 0xFFFF);
+            CT(CT_TRIAL) += 1;
+            px_trials++;
+            lmchisq = chi;
+            if (chi < chisq) {
+                if (C.regul_any) {
+                    const double pen = penalty_of(C, pt);
+                    if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
+                }
+                const bool conv = ((chisq - chi) / chisq) < C.convg;
+#pragma unroll
+                for (int k = 0; k < 9; k++) par[k] = pt[k];
+                chisq = chi;
+                if (conv) {
+                    ended = true;
+                } else {
+                    it++;
+                    if (it >= C.niter) ended = true;
+                    else phase = PH_JAC;
+                }
+            } else {
+                damp *= C.lrej;
+                if (damp > C.llim1) ended = true;
+            }
+        }
+        // ------------------------------------------- end of an LM run (H3, H9)
+        if (ended) {
+            const double crun = chisq / normp;
+            px_runs++;
+            px_iters += it;
+            CT(CT_RUNS) += 1;
+            CT(CT_ITERS) += it;
+            bool finish = false, restart = false;
+            if (lphase == 1) {
+                // learned start: accept at chisq_criteria, else heuristic runs
+                if (crun <= C.crit) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) SCR(SL_BEST + k) = par[k];
+                    SCR(SL_CHI_BEST) = crun;
+                    lphase = 0;
+                    finish = true;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) {
+                        SCR(SL_LEARN + k) = par[k];
+                        par[k] = SCR(SL_GUESS + k);
+                    }
+                    SCR(SL_CHI_LEARN) = crun;
+                    lphase = 2;
+                    px_status |= 4;
+                    CT(CT_FALLBACK) += 1;
+                    irun = 0;
+                    phase = PH_JAC_START;
+                }
+            } else {
+                bool save = false, brk = false;
+                double cbest = SCR(SL_CHI_BEST);
+                if (crun < C.crit) {
+                    save = true;
+                    brk = (irun >= 4 || irun == C.nruns * 4);
+                } else if (irun == 0) {
+                    save = true;
+                } else if (crun < cbest) {
+                    save = true;
+                }
+                if (save) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) SCR(SL_BEST + k) = par[k];
+                    SCR(SL_CHI_BEST) = crun;
+                    cbest = crun;
+                }
+                if (irun >= C.nruns * 2 && cbest < C.crit * 2) brk = true;
+                irun++;
+                if (!brk && irun < C.nruns * 4) restart = true;
+                else finish = true;
+            }
+            if (restart) {
+                // Random_Jump (Src/LM_FIT.c:157-201), Philox draws for irun >= 4
+                const int r = irun;
+                if (r < 4) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) par[k] = SCR(SL_BEST + k);
+                    par[2] = SCR(SL_GUESS + 2) + kPi / 4. * r;
+                } else {
+                    double nz[8];
+                    gauss4(C.seed, pix, (uint32_t)r, kStreamJump + 0, nz);
+                    gauss4(C.seed, pix, (uint32_t)r, kStreamJump + 1, nz + 4);
+                    par[0] = SCR(SL_GUESS + 0) * (1 + 0.6 * nz[0]);
+                    par[1] = SCR(SL_GUESS + 1) + kPi / 8. * nz[1];
+                    par[2] = SCR(SL_GUESS + 2) + kPi / 4. * (r + 0.3 * nz[2]);
+                    par[3] = SCR(SL_GUESS + 3) + 5. * nz[3];
+                    par[4] = SCR(SL_BEST + 4) * (1 + 0.2 * nz[4]);
+                    par[5] = SCR(SL_GUESS + 5) + 0.1 * nz[5];
+                    par[6] = SCR(SL_GUESS + 6) + 5 * nz[6];
+                    par[7] = SCR(SL_BEST + 7);
+                    par[8] = SCR(SL_GUESS + 8) + 0.1 * nz[7];
+                }
+                bounds_check(C, par, l7lo, l7hi);
+                phase = PH_JAC_START;
+            }
+            if (finish) {
+                double best[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) best[k] = SCR(SL_BEST + k);
+                double cbest = SCR(SL_CHI_BEST);
+                if (lphase == 2) {
+                    const double cl = SCR(SL_CHI_LEARN);
+                    if (cl < cbest) {
+                        cbest = cl;
+#pragma unroll
+                        for (int k = 0; k < 9; k++) best[k] = SCR(SL_LEARN + k);
+                    }
+                }
+                if (best[1] > kPi) best[1] -= kPi;
+                if (A.fitpar) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) A.fitpar[pix * 9 + k] = best[k];
+                }
+                if (C.hmi_ref) {
+                    best[2] += 0.5 * kPi;
+                    if (best[2] > kPi) best[2] -= kPi;
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++) A.res[pix * 10 + k] = best[k];
+                A.res[pix * 10 + 9] = cbest;
+                CT(CT_VALID) += 1;
+                if (cbest <= C.crit) CT(CT_CONV) += 1;
+                if (A.px_iters) {
+                    A.px_iters[pix] = px_iters;
+                    A.px_runs[pix] = px_runs;
+                    A.px_trials[pix] = px_trials;
+                    A.px_status[pix] = px_status;
+                }
+                phase = PH_ACQ;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------- counters
+#pragma unroll
+    for (int k = 0; k < CT_N; k++) {
+        unsigned long long v = CT(k);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (__lane_id() == 0 && v) atomicAdd(&A.counters[k], v);
+    }
+#undef CT
+#undef SCR
+}
+
+// forward synthesis, one pixel per thread
+template <typename T>
+__global__ __launch_bounds__(256) void synth_kernel(DevCfg C, const double *par, int64_t n, double *stokes, double *jac)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int nl = C.nl;
+    double pp[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) pp[k] = par[p * 9 + k];
+    PixK<T> K;
+    pix_consts<T>(C, pp, K);
+    for (int iw = 0; iw < nl; iw++) {
+        T o[4];
+        if (jac) {
+            me_point<T, true>(C, K, T(C.step[7]), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) jac[((p * 4 + s) * 9 + k) * nl + iw] = (double)row[k];
+            }, (FCount *)nullptr);
+        } else {
+            me_point<T, false>(C, K, T(C.step[7]), iw, o, NoSink{}, (FCount *)nullptr);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) stokes[(p * 4 + s) * nl + iw] = (double)o[s];
+    }
+}
+
+// best-fit profiles from the per-pixel parameters the inversion left in
+// fitpar; pixels that were not inverted (Doppler width 0) get zeros
+template <typename T>
+__global__ __launch_bounds__(256) void fit_kernel(DevCfg C, const double *fitpar, int64_t n, double *fit)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int nl = C.nl;
+    double pp[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) pp[k] = fitpar[p * 9 + k];
+    if (pp[4] == 0.0) {
+        for (int k = 0; k < 4 * nl; k++) fit[p * 4 * nl + k] = 0.0;
+        return;
+    }
+    PixK<T> K;
+    pix_consts<T>(C, pp, K);
+    for (int iw = 0; iw < nl; iw++) {
+        T o[4];
+        me_point<T, false>(C, K, T(0), iw, o, NoSink{}, (FCount *)nullptr);
+#pragma unroll
+        for (int s = 0; s < 4; s++) fit[(p * 4 + s) * nl + iw] = (double)o[s];
+    }
+}
+
+// synthetic map: Philox truths, FP64 synthesis, Gaussian noise sigma*S
+__global__ __launch_bounds__(256) void genmap_kernel(DevCfg C, uint64_t seed, const double *ranges, double ff_frac,
+                                                     double ff_bmax, double sigma, int64_t n, float *prof,
+                                                     double *truth)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int nl = C.nl;
+    uint32_t w[12];
+#pragma unroll
+    for (uint32_t b = 0; b < 3; b++) {
+        uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), b, kStreamTruth};
+        philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[4 * b + k] = c[k];
+    }
+    double t[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) t[k] = ranges[2 * k] + (ranges[2 * k + 1] - ranges[2 * k]) * u01(w[k]);
+    if (ff_frac > 0 && u01(w[9]) < ff_frac) t[0] = ff_bmax * u01(w[0]);
+    PixK<double> K;
+    pix_consts<double>(C, t, K);
+    // noise of (Stokes s, wavelength iw) is Box-Muller draw s of Philox block iw
+    for (int iw = 0; iw < nl; iw++) {
+        double gz[4], o[4];
+        gauss4(seed, p, (uint32_t)iw, kStreamNoise, gz);
+        me_point<double, false>(C, K, 0.0, iw, o, NoSink{}, (FCount *)nullptr);
+#pragma unroll
+        for (int s = 0; s < 4; s++) prof[p * 4 * nl + s * nl + iw] = (float)(o[s] + sigma * t[7] * gz[s]);
+    }
+    if (truth)
+#pragma unroll
+        for (int k = 0; k < 9; k++) truth[p * 9 + k] = t[k];
+}
+
+}  // namespace me

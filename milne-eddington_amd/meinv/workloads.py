@@ -1,0 +1,89 @@
+"""Synthetic inversion workloads: line, wavelength grid, truth ranges and the
+control-file settings each one runs with.
+
+Maps are drawn from Philox4x32-10 truths (seed 20261200 + index), synthesised
+by the FP64 forward model and given N(0, (sigma*S)^2) noise per Stokes
+sample; profiles are float32 in FITS order [n][4][nl].
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .abi import MeConfig, reference_defaults
+
+PI = 3.14159265358979323846
+SEED_BASE = 20261200
+
+
+@dataclass(frozen=True)
+class Workload:
+    name: str
+    index: int
+    lambda0: float      # Å
+    geff: float
+    nl: int
+    dlam: float         # Å between samples
+    nx: int
+    ny: int
+    dopp: tuple         # Doppler-width truth range [mÅ]
+    ff_frac: float = 0.0
+    ff_bmax: float = 50.0
+    sigma: float = 1e-3
+    num_run: int = 1
+    chisq_criteria: float = 1.5
+    bounds_dopp: tuple = (10.0, 80.0)
+
+    @property
+    def n(self) -> int:
+        return self.nx * self.ny
+
+    @property
+    def seed(self) -> int:
+        return SEED_BASE + self.index
+
+    def wavelengths(self) -> np.ndarray:
+        i = np.arange(self.nl, dtype=np.float64)
+        return self.lambda0 + (i - (self.nl - 1) / 2.0) * self.dlam
+
+    def ranges(self) -> np.ndarray:
+        return np.array([[0.0, 3000.0], [0.0, PI], [0.0, PI], [-3.0, 3.0], list(self.dopp),
+                         [0.0, 1.5], [1.0, 50.0], [0.5, 1.2], [0.1, 0.9]], dtype=np.float64)
+
+    def config(self, regul_theta: bool = False, **over) -> MeConfig:
+        """Control-file settings: weights 1, Icriteria 0.1 (continuum-normalised
+        maps), Damp/Eta/Beta/Dopp bounds covering the truth ranges."""
+        cfg = reference_defaults()
+        cfg.lambda0 = self.lambda0
+        cfg.geff = self.geff
+        cfg.num_run = self.num_run
+        cfg.chisq_criteria = self.chisq_criteria
+        cfg.icriteria = 0.1
+        cfg.bounds[4][0], cfg.bounds[4][1] = self.bounds_dopp
+        cfg.bounds[5][0], cfg.bounds[5][1] = 0.0, 1.5
+        cfg.bounds[6][0], cfg.bounds[6][1] = 1.0, 50.0
+        cfg.bounds[8][0], cfg.bounds[8][1] = 0.1, 0.9
+        if not regul_theta:
+            cfg.regul[1] = 0
+        cfg.seed = self.seed
+        for k, v in over.items():
+            setattr(cfg, k, v)
+        return cfg
+
+
+NIRIS = dict(lambda0=15648.5, geff=3.0, nl=60, dlam=0.08, dopp=(150.0, 350.0),
+             bounds_dopp=(100.0, 400.0))
+HINODE = dict(lambda0=6302.5, geff=2.5, nl=112, dlam=0.0215, dopp=(20.0, 50.0))
+HMI = dict(lambda0=6173.34, geff=2.5, nl=6, dlam=0.069, dopp=(20.0, 50.0))
+
+WORKLOADS = {
+    "niris64": Workload("niris64", 0, nx=64, ny=64, num_run=1, chisq_criteria=1.0, **NIRIS),
+    "niris1k": Workload("niris1k", 1, nx=1024, ny=1024, num_run=1, **NIRIS),
+    "hinode": Workload("hinode", 2, nx=1024, ny=512, num_run=1, **HINODE),
+    "hmi4k": Workload("hmi4k", 3, nx=4096, ny=4096, ff_frac=0.3, num_run=1, **HMI),
+}
+
+
+def get(name: str) -> Workload:
+    return WORKLOADS[name]

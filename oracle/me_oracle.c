@@ -1262,6 +1262,25 @@ static void orc_jump(orc_work *W, int64_t pix, int irun)
     orc_bounds_check(P, (const double(*)[2])W->lim);
 }
 
+/* save/break decision after run irun (Src/LM_FIT.c:713-724): a run under the
+ * criterion is saved even when worse than the best (H3); runs stop at
+ * irun >= 4 under the criterion or at irun >= 2*nruns with best < 2*criterion */
+static void orc_policy(int irun, int nruns, double crit, double chisq, double best, int *save, int *brk)
+{
+    *save = 0;
+    *brk = 0;
+    if (chisq < crit) {
+        *save = 1;
+        if (irun >= 4 || irun == nruns * 4) *brk = 1;
+    } else if (irun == 0) {
+        *save = 1;
+    } else if (chisq < best) {
+        *save = 1;
+    }
+    if (*save) best = chisq;
+    if (irun >= nruns * 2 && best < crit * 2) *brk = 1;
+}
+
 /* Src/LM_FIT.c:665-742; returns the run counter (0 = not inverted) */
 static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init, int32_t *runs)
 {
@@ -1289,17 +1308,10 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
         if (irun > 0) orc_jump(W, pix, irun);
         orc_lm_fit(W);
         (*runs)++;
-        if (W->chisq < cf->chisq_criteria) {
-            orc_sav(W);
-            if (irun >= 4 || irun == cf->num_run * 4) break;
-        } else if (irun == 0) {
-            orc_sav(W);
-        } else if (W->chisq < W->chisq_best) {
-            orc_sav(W);
-        }
-        if (irun >= cf->num_run * 2) {
-            if (W->chisq_best < cf->chisq_criteria * 2) break;
-        }
+        int save, brk;
+        orc_policy(irun, cf->num_run, cf->chisq_criteria, W->chisq, W->chisq_best, &save, &brk);
+        if (save) orc_sav(W);
+        if (brk) break;
     }
     if (W->Pb[1] > ORC_PI) W->Pb[1] -= ORC_PI;
     if (cf->hmi_ref) {
@@ -1317,17 +1329,10 @@ int orc_run_policy(int nruns, double criteria, const double *chisq_seq, int nseq
     for (irun = 0; irun < nruns * 4; irun++) {
         double c = irun < nseq ? chisq_seq[irun] : chisq_seq[nseq - 1];
         nexec++;
-        if (c < criteria) {
-            best = c; *best_run = irun;
-            if (irun >= 4 || irun == nruns * 4) break;
-        } else if (irun == 0) {
-            best = c; *best_run = irun;
-        } else if (c < best) {
-            best = c; *best_run = irun;
-        }
-        if (irun >= nruns * 2) {
-            if (best < criteria * 2) break;
-        }
+        int save, brk;
+        orc_policy(irun, nruns, criteria, c, best, &save, &brk);
+        if (save) { best = c; *best_run = irun; }
+        if (brk) break;
     }
     return nexec;
 }
@@ -1438,16 +1443,52 @@ void orc_gen_map(const orc_ctx *c, uint64_t seed, const double *ranges, double f
         for (int k = 0; k < 9; k++) t[k] = ranges[2 * k] + (ranges[2 * k + 1] - ranges[2 * k]) * orc_u01(w[k]);
         if (ff_frac > 0 && orc_u01(w[9]) < ff_frac) t[0] = ff_bmax * orc_u01(w[0]);
         orc_synth(c, t, c->cfg.step, syn, NULL, NULL);
-        for (int b = 0; b < nl; b++) {
+        /* noise of (Stokes s, wavelength iw) is Box-Muller draw s of block iw */
+        for (int iw = 0; iw < nl; iw++) {
             double g[4];
-            orc_gauss4(seed, p, (uint32_t)b, ORC_STREAM_NOISE, g);
-            for (int k = 0; k < 4; k++) {
-                int idx = 4 * b + k;
-                prof[4 * nl * i + idx] = (float)(syn[idx] + sigma * t[7] * g[k]);
-            }
+            orc_gauss4(seed, p, (uint32_t)iw, ORC_STREAM_NOISE, g);
+            for (int s = 0; s < 4; s++)
+                prof[4 * nl * i + s * nl + iw] = (float)(syn[s * nl + iw] + sigma * t[7] * g[s]);
         }
         if (truth)
             for (int k = 0; k < 9; k++) truth[9 * i + k] = t[k];
     }
     free(syn);
+}
+
+/* ======================================================================== */
+/* probes of single LM stages for tests                                      */
+/* ======================================================================== */
+
+/* normal matrix, gradient and chi2 at par for one pixel, as Hessian_Compute
+ * and Loss_Function build them (no regularisation) */
+void orc_normal_eq(const orc_ctx *c, const double *par, const double *prof, double step7, double norm0,
+                   double *H, double *g, double *chi2)
+{
+    orc_work W;
+    double *mem = (double *)malloc(sizeof(double) * (48 * c->nl + 8));
+    orc_work_init(&W, c, mem);
+    W.step[7] = step7;
+    W.prof = (double *)prof;
+    for (int s = 0; s < 4; s++) W.iv[s] = c->wsq[s] * (1. / norm0);
+    orc_synth(c, par, W.step, W.syn, W.jac, NULL);
+    orc_hessian(&W);
+    *chi2 = orc_loss(&W);
+    memcpy(H, W.H, sizeof(double) * W.np * W.np);
+    memcpy(g, W.g, sizeof(double) * W.np);
+    free(mem);
+}
+
+/* truncated SVD solve of SVD_solve_mem; H is replaced by U (H1) */
+void orc_solve(const orc_ctx *c, double *H, const double *g, double *sol)
+{
+    orc_work W;
+    double *mem = (double *)malloc(sizeof(double) * (48 * c->nl + 8));
+    orc_work_init(&W, c, mem);
+    memcpy(W.H, H, sizeof(double) * W.np * W.np);
+    memcpy(W.g, g, sizeof(double) * W.np);
+    orc_svd_solve(&W);
+    memcpy(H, W.H, sizeof(double) * W.np * W.np);
+    memcpy(sol, W.sol, sizeof(double) * W.np);
+    free(mem);
 }

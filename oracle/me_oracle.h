@@ -111,6 +111,12 @@ typedef struct orc_pixstats {
 void orc_invert(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
                 const double *init, int h2_mode, int nthreads, orc_pixstats *pst, int64_t *ev);
 
+/* single LM stages: normal equations at par (no regularisation), and the
+ * truncated SVD solve that overwrites H with U */
+void orc_normal_eq(const orc_ctx *c, const double *par, const double *prof, double step7, double norm0,
+                   double *H, double *g, double *chi2);
+void orc_solve(const orc_ctx *c, double *H, const double *g, double *sol);
+
 /* Synthetic map: same sampler and Philox streams as the device generator.
  * ranges [9][2]; writes prof [n][4][nl] (float) and truth [n][9]. */
 void orc_gen_map(const orc_ctx *c, uint64_t seed, const double *ranges, double ff_frac,

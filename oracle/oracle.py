@@ -64,6 +64,8 @@ def lib():
                                  C.POINTER(PixStats), C.POINTER(C.c_int64)]
         L.orc_gen_map.argtypes = [C.c_void_p, C.c_uint64, _dp, C.c_double, C.c_double, C.c_double,
                                   C.c_int64, C.c_int64, _fp, _dp]
+        L.orc_normal_eq.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_run_policy.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _ip]
         L.orc_run_policy.restype = C.c_int
         _lib = L
@@ -222,6 +224,23 @@ class Oracle:
         lib().orc_synth(self.h, _ptr(par, _dp), _ptr(st, _dp), _ptr(syn, _dp), None,
                         cnt.ctypes.data_as(C.POINTER(C.c_int64)))
         return syn, cnt
+
+    def normal_eq(self, par, prof, step7, norm0):
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        prof = np.ascontiguousarray(prof, dtype=np.float64).reshape(4, self.nl)
+        H = np.empty((9, 9))
+        g = np.empty(9)
+        chi = C.c_double()
+        lib().orc_normal_eq(self.h, _ptr(par, _dp), _ptr(prof, _dp), step7, norm0, _ptr(H, _dp),
+                            _ptr(g, _dp), C.byref(chi))
+        return H, g, chi.value
+
+    def solve(self, H, g):
+        H = np.array(H, dtype=np.float64, order="C")
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        sol = np.empty(9)
+        lib().orc_solve(self.h, _ptr(H, _dp), _ptr(g, _dp), _ptr(sol, _dp))
+        return sol, H
 
     def init_guess(self, prof_batch, pix=0):
         prof_batch = np.ascontiguousarray(prof_batch, dtype=np.float64).reshape(-1, 4, self.nl)
