@@ -114,6 +114,13 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise RuntimeError(f"libmeinv.so not built at {p}; run __graft_entry__.build()")
+    # torch ships its own libamdhip64.so with the same soname: when both are
+    # used in one process, load torch's first so that one HIP runtime serves
+    # both (the other order leaves torch without devices)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, res, args in ENTRY_POINTS:
         fn = getattr(lib, name)

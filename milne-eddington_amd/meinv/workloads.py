@@ -68,7 +68,12 @@ class Workload:
             cfg.regul[1] = 0
         cfg.seed = self.seed
         for k, v in over.items():
-            setattr(cfg, k, v)
+            if isinstance(v, (tuple, list)):
+                arr = getattr(cfg, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(cfg, k, v)
         return cfg
 
 

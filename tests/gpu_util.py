@@ -2,6 +2,11 @@
 only); every computation under test goes through libmeinv.so's C ABI."""
 import numpy as np
 
+try:  # one HIP runtime per process: torch's must load before libmeinv's
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 import oracle as O
 from meinv import Inversion
 from meinv.workloads import get
@@ -39,3 +44,25 @@ def within_parity(a, b):
 def gpu_inversion(name, **over):
     w = get(name)
     return Inversion(w.config(**over), w.wavelengths())
+
+
+def within_physical(a, b, dB=0.1, ddeg=0.01, dv=1e-3):
+    """Agreement on the four quantities of the stated bar only."""
+    ok = np.abs(a[:, 0] - b[:, 0]) <= dB
+    ok &= np.abs(a[:, 1] - b[:, 1]) * DEG <= ddeg
+    dphi = np.abs(a[:, 2] - b[:, 2])
+    dphi = np.minimum(dphi, np.pi - dphi)
+    ok &= dphi * DEG <= ddeg
+    ok &= np.abs(a[:, 3] - b[:, 3]) <= dv
+    return ok
+
+
+def chaos_floor(orc, prof64, init=None, rel=1e-15, seed=0):
+    """Agreement of the oracle with itself when the input moves by ~1 ulp:
+    later LM trials solve with the left factor of an orthogonal matrix's SVD
+    (H1), which rounding alone decides, so no second implementation can agree
+    more often than this."""
+    a = orc.invert(prof64, init=init, nthreads=8)
+    rng = np.random.default_rng(seed)
+    b = orc.invert(prof64 * (1 + rel * rng.standard_normal(prof64.shape)), init=init, nthreads=8)
+    return within_parity(b["res"], a["res"]).mean(), a
