@@ -46,6 +46,7 @@ struct me_ctx {
     double *d_lam = nullptr, *d_dlam = nullptr, *d_etab = nullptr;
     float *d_dlamf = nullptr, *d_etabf = nullptr;
     unsigned long long *d_queue = nullptr, *d_counters = nullptr;
+    DevCfg *d_cfg = nullptr;  // device copy of dc, read by the inversion kernel
     DevBuf scratch, prof, res, fit, init, pxs, par, out, jac, fitpar;
     me::MlpState mlp;
 };
@@ -123,7 +124,7 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
     A.px_trials = d_px ? d_px + 2 * n : nullptr;
     A.px_status = d_px ? d_px + 3 * n : nullptr;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, c->dc, A);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (d_fit) {
@@ -377,6 +378,12 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     d.dlamf = c->d_dlamf;
     d.etab = c->d_etab;
     d.etabf = c->d_etabf;
+    e = hipMalloc(&c->d_cfg, sizeof(DevCfg));
+    e = e ? e : hipMemcpy(c->d_cfg, &d, sizeof(DevCfg), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        me_ctx_destroy(c);
+        return hip_status(e);
+    }
     me::mlp_init(c->mlp, 4 * nl);
     *out = c;
     return ME_OK;
@@ -393,6 +400,7 @@ void me_ctx_destroy(me_ctx *c)
     hipFree(c->d_etabf);
     hipFree(c->d_queue);
     hipFree(c->d_counters);
+    hipFree(c->d_cfg);
     DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar};
     for (DevBuf *b : bufs) b->release();
     me::mlp_free(c->mlp);

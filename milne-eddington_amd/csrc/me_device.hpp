@@ -561,6 +561,10 @@ __device__ __forceinline__ double hyp(double a, double b)
     return mx * sqrt(1.0 + r * r);
 }
 
+// keeps the scheduler from hoisting work across SVD steps (live ranges of
+// the unrolled code would otherwise outgrow the register file)
+#define SVD_BARRIER() __builtin_amdgcn_sched_barrier(0)
+
 // compile-time loop: f(integral_constant<int, I>) for I = B .. E-1 (ascending)
 // or E-1 .. B (descending), so that every index derived from I is a constant
 template <int B, int E, typename F> __device__ __forceinline__ void sfor_up(F &&f)
@@ -621,6 +625,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 double fc = d / h;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += fc * a[k * 9 + i];
+                SVD_BARRIER();
             });
 #pragma unroll
             for (int k = i; k < 9; k++) a[k * 9 + i] *= scale;
@@ -652,6 +657,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                     for (int k = l; k < 9; k++) d += a[j * 9 + k] * a[i * 9 + k];
 #pragma unroll
                     for (int k = l; k < 9; k++) a[j * 9 + k] += d * rv[k];
+                    SVD_BARRIER();
                 });
 #pragma unroll
                 for (int k = l; k < 9; k++) a[i * 9 + k] *= scale;
@@ -677,6 +683,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 double f = (d / a[i * 9 + i]) * gg;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += f * a[k * 9 + i];
+                SVD_BARRIER();
             });
 #pragma unroll
             for (int j = i; j < 9; j++) a[j * 9 + i] *= gg;
@@ -799,6 +806,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         a[r * 9 + i] = z2 * c - y2 * s;
                     }
                 }
+                SVD_BARRIER();
             }
 #pragma unroll
             for (int kk = 0; kk < 9; kk++) {

@@ -47,10 +47,22 @@ template <typename TIn> __device__ __forceinline__ double ld(const void *p, int6
 }
 
 // Init_Guess of Src/ME_SOLVER.c:445-676 on the lane's pixel (FP64).
+struct P9 {
+    double v[9];
+};
+
+struct Guess {
+    double g[9];
+    double l7lo, l7hi, step7, norm0, normp;
+};
+
 template <typename TIn>
-__device__ void init_guess(const DevCfg &C, const void *prof, int64_t pix, double g[9], double &l7lo, double &l7hi,
-                           double &step7, double &norm0, double &normp)
+__device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t pix)
 {
+    const DevCfg &C = *Cp;
+    Guess G;
+    double *g = G.g;
+    double &l7lo = G.l7lo, &l7hi = G.l7hi, &step7 = G.step7, &norm0 = G.norm0, &normp = G.normp;
     const int nl = C.nl;
     const int64_t base = pix * 4 * nl;
     double sI = 0., Imin = 1e10, sQ = 0., sU = 0., sV = 0.;
@@ -144,6 +156,7 @@ __device__ void init_guess(const DevCfg &C, const void *prof, int64_t pix, doubl
     l7lo = Imax * 0.6;
     l7hi = Imax * 1.4;
     step7 = Imax * 0.1;
+    return G;
 }
 
 __device__ __forceinline__ double penalty_of(const DevCfg &C, const double p[9])
@@ -196,9 +209,158 @@ template <typename T> struct HessSink {
     }
 };
 
+// Synthesis with Jacobian at par for one pixel, folded into the 9x9 normal
+// matrix (written to mat, strided by NS, upper and lower triangle) and the
+// gradient (gout); returns chi2 and the packed Faddeeva branch counts.
+struct PassRes {
+    double chi;
+    uint64_t flo, fhi;
+};
+
 template <typename T, typename TIn>
-__global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
+__device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
+                                         P9 par, double step7, double ivn, double *__restrict__ mat,
+                                         double *__restrict__ gout, int64_t NS)
 {
+    const DevCfg &C = *Cp;
+    const int nl = C.nl;
+    PixK<T> K;
+    pix_consts<T>(C, par.v, K);
+    const T ivT[4] = {T(C.wsq[0] * ivn), T(C.wsq[1] * ivn), T(C.wsq[2] * ivn), T(C.wsq[3] * ivn)};
+    T hacc[45], gacc[9];
+#pragma unroll
+    for (int k = 0; k < 45; k++) hacc[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) gacc[k] = 0;
+    double chi = 0.0;
+    FCount fc;
+    for (int iw = 0; iw < nl; iw++) {
+        T o[4], d[4];
+        HessSink<T> sink{hacc, gacc, d, ivT};
+        me_point<T, true>(C, K, T(step7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+            if (s == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    d[q] = o[q] - T(ld<TIn>(prof, base + q * nl + iw));
+                    chi += (double)d[q] * (double)d[q] * (C.wsq[q] * ivn);
+                }
+            }
+            sink(s, row);
+        }, &fc);
+    }
+    int idx = 0;
+#pragma unroll
+    for (int a = 0; a < 9; a++) {
+#pragma unroll
+        for (int b = a; b < 9; b++) {
+            mat[(int64_t)(a * 9 + b) * NS] = (double)hacc[idx];
+            if (b != a) mat[(int64_t)(b * 9 + a) * NS] = (double)hacc[idx];
+            idx++;
+        }
+        gout[(int64_t)a * NS] = (double)gacc[a];
+    }
+    return PassRes{chi / (4. * nl), fc.lo, fc.hi};
+}
+
+// Trial synthesis at pt: chi2 only (Loss_Function of Src/LM_FIT.c:89-120)
+template <typename T, typename TIn>
+__device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
+                                           P9 pt, double ivn)
+{
+    const DevCfg &C = *Cp;
+    const int nl = C.nl;
+    PixK<T> K;
+    pix_consts<T>(C, pt.v, K);
+    double chi = 0.0;
+    FCount fc;
+    for (int iw = 0; iw < nl; iw++) {
+        T o[4];
+        me_point<T, false>(C, K, T(0), iw, o, NoSink{}, &fc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const T d = o[q] - T(ld<TIn>(prof, base + q * nl + iw));
+            chi += (double)d * (double)d * (C.wsq[q] * ivn);
+        }
+    }
+    return PassRes{chi / (4. * nl), fc.lo, fc.hi};
+}
+
+// Truncated SVD solve of SVD_solve_mem (Src/LM_FIT.c:400-469) for one lane.
+// in: the 9x9 input matrix A (strided by NS), U is written to uo (H1: it is
+// the next trial's matrix), g the gradient.  x = V W^+ U^T g is formed as
+// A^T U W^-1 W^+ U^T g; the singular-value threshold grows x5 until
+// |x| <= 0.5, else x is clipped.  Not inlined: the SVD gets the register file
+// to itself instead of sharing it with the rest of the LM state machine.
+struct Sol9 {
+    double x[9];
+    int status, nbk;
+};
+
+__device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
+                                       const double *__restrict__ gv, int64_t NS, double thr0)
+{
+    Sol9 r;
+    double M[81], w[9];
+#pragma unroll
+    for (int k = 0; k < 81; k++) M[k] = in[(int64_t)k * NS];
+    r.status = svd9(M, w);
+    double ug[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        double sacc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) sacc += M[i * 9 + j] * gv[(int64_t)i * NS];
+        ug[j] = sacc;
+    }
+#pragma unroll
+    for (int k = 0; k < 81; k++) uo[(int64_t)k * NS] = M[k];
+    double wmax = w[0];
+#pragma unroll
+    for (int i = 1; i < 9; i++) wmax = wmax > w[i] ? wmax : w[i];
+    double thr = thr0;
+    r.nbk = 0;
+    for (int pass = 0; pass < 64; pass++) {
+        const double wt = wmax * thr;
+        double y[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            if (w[j] < wt) w[j] = 0.;
+            y[j] = w[j] > 1e-50 ? (ug[j] / w[j]) / w[j] : 0.0;
+        }
+        double uy[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 9; j++) sacc += M[k * 9 + j] * y[j];
+            uy[k] = sacc;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) sacc += in[(int64_t)(k * 9 + i) * NS] * uy[k];
+            r.x[i] = sacc;
+        }
+        r.nbk++;
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < 9; i++) ok = ok && !(r.x[i] > 0.5 || r.x[i] < -0.5);
+        if (ok) break;
+        if (thr > 2e-3) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) r.x[i] = r.x[i] > 0.5 ? 0.5 : (r.x[i] < -0.5 ? -0.5 : r.x[i]);
+            break;
+        }
+        thr *= 5.;
+    }
+    return r;
+}
+
+template <typename T, typename TIn>
+__global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
+{
+    const DevCfg &C = *Cp;
     const int nl = C.nl;
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *scr = A.scratch + slot;
@@ -249,8 +411,14 @@ __global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
                 }
                 continue;  // stays PH_ACQ
             }
-            double gs[9], norm0;
-            init_guess<TIn>(C, A.prof, pix, gs, l7lo, l7hi, step7, norm0, normp);
+            const Guess G = init_guess<TIn>(Cp, A.prof, pix);
+            double gs[9], norm0 = G.norm0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) gs[k] = G.g[k];
+            l7lo = G.l7lo;
+            l7hi = G.l7hi;
+            step7 = G.step7;
+            normp = G.normp;
             if (A.init && !A.learned) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) gs[k] = A.init[pix * 9 + k];
@@ -279,39 +447,19 @@ __global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
 
         bool ended = false;
         const int64_t base = pix * 4 * nl;
-        const T ivT[4] = {T(C.wsq[0] * ivn), T(C.wsq[1] * ivn), T(C.wsq[2] * ivn), T(C.wsq[3] * ivn)};
 
         // ---------------------------------------------- Jacobian + normal matrix
         if (phase == PH_JAC_START || phase == PH_JAC) {
-            PixK<T> K;
-            pix_consts<T>(C, par, K);
-            T hacc[45], gacc[9];
+            P9 pv;
 #pragma unroll
-            for (int k = 0; k < 45; k++) hacc[k] = 0;
+            for (int k = 0; k < 9; k++) pv.v[k] = par[k];
+            double *mat = scr + (int64_t)(SL_MAT + 81 * mbuf) * NS;
+            const PassRes jr = jac_pass<T, TIn>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
+            const double chi = jr.chi;
 #pragma unroll
-            for (int k = 0; k < 9; k++) gacc[k] = 0;
-            double chi = 0.0;
-            FCount fc;
-            for (int iw = 0; iw < nl; iw++) {
-                T o[4], d[4];
-                // the sink reads d, which is filled before the first row arrives
-                HessSink<T> sink{hacc, gacc, d, ivT};
-                me_point<T, true>(C, K, T(step7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
-                    if (s == 0) {
+            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            d[q] = o[q] - T(ld<TIn>(A.prof, base + q * nl + iw));
-                            chi += (double)d[q] * (double)d[q] * (C.wsq[q] * ivn);
-                        }
-                    }
-                    sink(s, row);
-                }, &fc);
-            }
-            chi /= (4. * nl);
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((fc.lo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((fc.hi >> (16 * b)) & 0xFFFF);
+            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
             CT(CT_JAC) += 1;
             CT(CT_HESS) += 1;
             if (phase == PH_JAC_START) {
@@ -323,41 +471,17 @@ __global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
             }
-            // normal matrix (+ regularisation, Src/LM_FIT.c:373-393) into the
-            // input slot of the trial, gradient into SL_G
-            double gg[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) gg[k] = (double)gacc[k];
-            double diag[9];
-            {
-                int idx = 0;
-#pragma unroll
-                for (int a = 0; a < 9; a++) {
-#pragma unroll
-                    for (int b = a; b < 9; b++) {
-                        if (b == a) diag[a] = (double)hacc[idx];
-                        else {
-                            SCR(SL_MAT + 81 * mbuf + a * 9 + b) = (double)hacc[idx];
-                            SCR(SL_MAT + 81 * mbuf + b * 9 + a) = (double)hacc[idx];
-                        }
-                        idx++;
-                    }
-                }
-            }
+            // regularisation terms on the diagonal and the gradient
+            // (Src/LM_FIT.c:373-393)
             if (C.regul_any) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) {
                     if (C.regul[k]) {
                         const double r = par[k] - C.rval[k];
-                        gg[k] -= (r * C.rw[k]) * ratio;
-                        diag[k] += C.rw[k] * ratio;
+                        SCR(SL_G + k) -= (r * C.rw[k]) * ratio;
+                        mat[(int64_t)(k * 10) * NS] += C.rw[k] * ratio;
                     }
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                SCR(SL_MAT + 81 * mbuf + k * 10) = diag[k];
-                SCR(SL_G + k) = gg[k];
             }
             damp = C.damp0;
             if (phase == PH_JAC_START && C.niter <= 0) ended = true;
@@ -369,89 +493,29 @@ __global__ __launch_bounds__(256) void invert_kernel(DevCfg C, InvArgs A)
             // SVD of the input matrix (the normal matrix, or U of the previous
             // rejected trial: H1); U goes to the other slot and becomes the
             // next trial's input
-            double M[81], w[9];
-            double *in = scr + (int64_t)(SL_MAT + 81 * mbuf) * NS;
-            double *uo = scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS;
-#pragma unroll
-            for (int k = 0; k < 81; k++) M[k] = in[(int64_t)k * NS];
-            const int sf = svd9(M, w);
+            const Sol9 sv = svd_solve(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
+                                      scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr);
             CT(CT_SVD) += 1;
-            if (sf) { px_status |= 2; CT(CT_SVDFAIL) += 1; }
-            double ug[9];
+            CT(CT_BK) += sv.nbk;
+            if (sv.status) { px_status |= 2; CT(CT_SVDFAIL) += 1; }
+            double sol[9];
 #pragma unroll
-            for (int j = 0; j < 9; j++) {
-                double sacc = 0.0;
-#pragma unroll
-                for (int i = 0; i < 9; i++) sacc += M[i * 9 + j] * SCR(SL_G + i);
-                ug[j] = sacc;
-            }
-#pragma unroll
-            for (int k = 0; k < 81; k++) uo[(int64_t)k * NS] = M[k];
-            double wmax = w[0];
-#pragma unroll
-            for (int i = 1; i < 9; i++) wmax = wmax > w[i] ? wmax : w[i];
-            // truncated solve (Src/LM_FIT.c:423-466): x = V W^+ U^T g with
-            // V W^+ t = A^T U W^-1 W^+ t; thresholds x5 until |x| <= 0.5
-            double thr = C.thr, sol[9];
-            for (int pass = 0; pass < 64; pass++) {
-                const double wt = wmax * thr;
-                double y[9];
-#pragma unroll
-                for (int j = 0; j < 9; j++) {
-                    if (w[j] < wt) w[j] = 0.;
-                    y[j] = w[j] > 1e-50 ? (ug[j] / w[j]) / w[j] : 0.0;
-                }
-                double uy[9];
-#pragma unroll
-                for (int k = 0; k < 9; k++) {
-                    double sacc = 0.0;
-#pragma unroll
-                    for (int j = 0; j < 9; j++) sacc += M[k * 9 + j] * y[j];
-                    uy[k] = sacc;
-                }
-#pragma unroll
-                for (int i = 0; i < 9; i++) {
-                    double sacc = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 9; k++) sacc += in[(int64_t)(k * 9 + i) * NS] * uy[k];
-                    sol[i] = sacc;
-                }
-                CT(CT_BK) += 1;
-                bool ok = true;
-#pragma unroll
-                for (int i = 0; i < 9; i++) ok = ok && !(sol[i] > 0.5 || sol[i] < -0.5);
-                if (ok) break;
-                if (thr > 2e-3) {
-#pragma unroll
-                    for (int i = 0; i < 9; i++) sol[i] = sol[i] > 0.5 ? 0.5 : (sol[i] < -0.5 ? -0.5 : sol[i]);
-                    break;
-                }
-                thr *= 5.;
-            }
+            for (int k = 0; k < 9; k++) sol[k] = sv.x[k];
             mbuf ^= 1;
             double pt[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) pt[k] = par[k] + sol[k] * (k == 7 ? step7 : C.step[k]);
             bounds_check(C, pt, l7lo, l7hi);
 
-            PixK<T> K;
-            pix_consts<T>(C, pt, K);
-            double chi = 0.0;
-            FCount fc;
-            for (int iw = 0; iw < nl; iw++) {
-                T o[4];
-                me_point<T, false>(C, K, T(step7), iw, o, NoSink{}, &fc);
+            P9 pv;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const T d = o[q] - T(ld<TIn>(A.prof, base + q * nl + iw));
-                    chi += (double)d * (double)d * (C.wsq[q] * ivn);
-                }
-            }
-            chi /= (4. * nl);
+            for (int k = 0; k < 9; k++) pv.v[k] = pt[k];
+            const PassRes tr = trial_pass<T, TIn>(Cp, A.prof, base, pv, ivn);
+            const double chi = tr.chi;
 #pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((fc.lo >> (16 * b)) & 0xFFFF);
+            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((tr.flo >> (16 * b)) & 0xFFFF);
 #pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((fc.hi >> (16 * b)) & 0xFFFF);
+            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((tr.fhi >> (16 * b)) & 0xFFFF);
             CT(CT_TRIAL) += 1;
             px_trials++;
             lmchisq = chi;
