@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
+    ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
+    ap.add_argument("--train-epochs", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -93,7 +96,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from meinv import ME_FP32, ME_FP64, Inversion
+    from meinv import ME_FP32, ME_FP64, ME_INIT_LEARNED, Inversion
     from meinv.workloads import get
 
     w = get(args.workload)
@@ -109,29 +112,63 @@ def main():
     inv.gen_map_device(w.seed, w.ranges(), n, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
                        sigma=w.sigma)
     torch.cuda.synchronize()
-    print(f"bench: {w.name} n={n} nl={w.nl} {args.precision} num_run={cfg.num_run}", file=sys.stderr, flush=True)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    log(f"bench: {w.name} n={n} nl={w.nl} {args.precision} num_run={cfg.num_run} mode={args.mode}")
 
-    for i in range(args.warmup):
-        st = inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
-        print(f"warmup {i}: {st.kernel_ms:.1f} ms, converged {st.n_converged}/{st.n_valid}", file=sys.stderr,
-              flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = []
-    for i in range(args.steps):
-        st = inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
-        stats.append(st)
-        print(f"step {i}: {st.kernel_ms:.1f} ms, converged {st.n_converged}/{st.n_valid}", file=sys.stderr,
-              flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    last = stats[-1]
+    def timed(engine, label):
+        for i in range(args.warmup):
+            st = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+            log(f"{label} warmup {i}: {st.total_ms:.1f} ms (kernel {st.kernel_ms:.1f}), converged "
+                f"{st.n_converged}/{st.n_valid}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = []
+        for i in range(args.steps):
+            st = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+            sts.append(st)
+            log(f"{label} step {i}: {st.total_ms:.1f} ms (kernel {st.kernel_ms:.1f}), converged "
+                f"{st.n_converged}/{st.n_valid}")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall / args.steps * 1e3, sts
+
+    def summary(ms, sts):
+        last = sts[-1]
+        v = max(1, last.n_valid)
+        return {"converged_pixels_per_s": round(last.n_converged / (ms / 1e3), 1), "ms_per_step": round(ms, 2),
+                "converged": int(last.n_converged), "valid": int(last.n_valid),
+                "converged_fraction": round(last.n_converged / v, 4),
+                "runs_per_pixel": round(last.n_runs / v, 3), "iters_per_pixel": round(last.n_iters / v, 3),
+                "trials_per_pixel": round(last.n_trial / v, 3), "fallback_fraction": round(last.n_fallback / v, 4),
+                "kernel_ms": round(float(np.mean([s.kernel_ms for s in sts])), 2), "svd_fail": int(last.n_svd_fail)}
+
+    results = {}
+    plain_ms = plain_sts = None
+    if args.mode in ("plain", "both"):
+        plain_ms, plain_sts = timed(inv, "plain")
+        results["plain"] = summary(plain_ms, plain_sts)
+    learned_ms = learned_sts = None
+    train_s = None
+    if args.mode in ("learned", "both"):
+        linv = Inversion(w.config(init_mode=ME_INIT_LEARNED, **over), w.wavelengths())
+        t = time.perf_counter()
+        loss = linv.mlp_train(w.seed + 100, w.ranges(), args.train_samples, batch=4096, epochs=args.train_epochs,
+                              lr=2e-3, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
+        train_s = time.perf_counter() - t
+        log(f"MLP trained on {args.train_samples} device-synthesised profiles x {args.train_epochs} epochs in "
+            f"{train_s:.1f} s, final loss {loss:.4g}")
+        learned_ms, learned_sts = timed(linv, "learned")
+        results["learned"] = summary(learned_ms, learned_sts)
+        results["learned"]["train_s"] = round(train_s, 2)
+        results["learned"]["train_loss"] = loss
+    headline = "learned" if learned_sts is not None else "plain"
+    ms_per_step = learned_ms if headline == "learned" else plain_ms
+    sts = learned_sts if headline == "learned" else plain_sts
+    last = sts[-1]
     conv = last.n_converged
     value = conv / (ms_per_step / 1e3)
-    kernel_s = np.mean([s.kernel_ms for s in stats]) / 1e3
+    kernel_s = np.mean([s.kernel_ms for s in sts]) / 1e3
     res = d_res.cpu().numpy()
-
     threads = min(os.cpu_count() or 1, 16)
     cpu = cpu_baseline(w, cfg, args.cpu_seconds, threads) if args.cpu_seconds > 0 else None
 
@@ -150,18 +187,14 @@ def main():
         "data": "synthetic (Philox truths, FP64 forward model, noise 1e-3 S)",
         "config": {"workload": f"{w.name}: Fe I {w.lambda0} A, nl={w.nl}, {w.nx}x{w.ny} px"
                                + (f", {int(w.ff_frac * 100)}% field-free" if w.ff_frac else ""),
-                   "pixels": n, "init": "heuristic (plain LM)", "num_run": cfg.num_run,
-                   "chisq_criteria": cfg.chisq_criteria, "precision": args.precision},
+                   "pixels": n, "init": "learned MLP + LM, fallback plain" if headline == "learned" else "heuristic",
+                   "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision},
         "roofline": roofline(last, w.nl, fp64, kernel_s),
         "cpu_baseline": cpu,
-        "details": {"converged": int(conv), "valid": int(last.n_valid),
-                    "converged_fraction": round(conv / max(1, last.n_valid), 4),
-                    "runs_per_pixel": round(last.n_runs / max(1, last.n_valid), 3),
-                    "iters_per_pixel": round(last.n_iters / max(1, last.n_valid), 3),
-                    "trials_per_pixel": round(last.n_trial / max(1, last.n_valid), 3),
-                    "svd_per_pixel": round(last.n_svd / max(1, last.n_valid), 3),
-                    "kernel_ms": round(kernel_s * 1e3, 2), "svd_fail": int(last.n_svd_fail),
-                    "median_chi2": float(np.median(res[:, 9]))},
+        "details": dict(results, median_chi2=float(np.median(res[:, 9])),
+                        speedup_learned_vs_plain=(round(results["learned"]["converged_pixels_per_s"]
+                                                        / results["plain"]["converged_pixels_per_s"], 3)
+                                                  if "learned" in results and "plain" in results else None)),
     }
     print(json.dumps(line), flush=True)
 

@@ -526,9 +526,23 @@ me_status me_gen_map_device(me_ctx *c, uint64_t seed, const double *ranges, doub
 me_status me_mlp_train(me_ctx *c, uint64_t seed, const double *ranges, double ff_frac, double ff_bmax, double sigma,
                        int64_t n_train, int batch, int epochs, double lr, double *final_loss)
 {
-    if (!c || !ranges || n_train <= 0 || batch <= 0 || epochs <= 0) return ME_EINVAL;
-    return me::mlp_train(c->mlp, c->dc, c->stream, seed, ranges, ff_frac, ff_bmax, sigma, n_train, batch, epochs, lr,
-                         final_loss);
+    if (!c || !ranges || n_train < 64 || batch <= 0 || epochs <= 0 || !(lr > 0)) return ME_EINVAL;
+    double *d_rg = nullptr;
+    HIPCHK(hipMalloc((void **)&d_rg, sizeof(double) * 18));
+    HIPCHK(hipMemcpy(d_rg, ranges, sizeof(double) * 18, hipMemcpyHostToDevice));
+    // the training set: profiles synthesised on the device from the truth ranges
+    auto gen = [&](float *prof, double *truth, int64_t n) -> me_status {
+        const int block = 256;
+        const int grid = (int)((n + block - 1) / block);
+        hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, c->dc, seed, (const double *)d_rg,
+                           ff_frac, ff_bmax, sigma, n, prof, truth);
+        return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
+    };
+    me_status s = me::mlp_train_impl(c->mlp, c->dc, c->stream, seed, ranges, n_train, batch, epochs, lr, final_loss,
+                                     gen);
+    hipStreamSynchronize(c->stream);
+    hipFree(d_rg);
+    return s;
 }
 
 me_status me_mlp_predict_device(me_ctx *c, const float *d_prof, int64_t n, double *d_init)
@@ -536,7 +550,9 @@ me_status me_mlp_predict_device(me_ctx *c, const float *d_prof, int64_t n, doubl
     if (!c || n < 0 || (n > 0 && (!d_prof || !d_init))) return ME_EINVAL;
     if (!c->mlp.trained) return ME_ENOTREADY;
     if (n == 0) return ME_OK;
-    return me::mlp_predict(c->mlp, c->dc, c->stream, d_prof, 0, n, d_init);
+    me_status s = me::mlp_predict(c->mlp, c->dc, c->stream, d_prof, 0, n, d_init);
+    if (s == ME_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = ME_EDEVICE;
+    return s;
 }
 
 int64_t me_mlp_num_params(const me_ctx *c) { return c ? me::mlp_num_params(c->mlp) : 0; }

@@ -1,33 +1,559 @@
-// me_mlp.hpp — learned initialiser (MLP 4*nl -> 256 -> 256 -> 256 -> 10).
+// me_mlp.hpp — learned initialiser: MLP 4*nl -> 256 -> 256 -> 256 -> 10
+// (GELU), BF16 operands on MFMA (v_mfma_f32_16x16x32_bf16) with FP32
+// accumulation, FP32 master weights and Adam.  Trained on profiles the
+// device map generator synthesises from the truth ranges; its prediction
+// seeds one LM run per pixel (init_mode = ME_INIT_LEARNED).
+//
+// Normalisation: inputs are I/Imax - 1 and 10*Q/Imax, 10*U/Imax, 10*V/Imax
+// (4*nl values, padded to a multiple of 32); outputs are the truths mapped to
+// [0,1] by their ranges, except phi (sin 2phi, cos 2phi) and S (S/Imax).
 #pragma once
+
+#include <hip/hip_bf16.h>
+
+#include <cstring>
+#include <vector>
 
 #include "../../include/meinv.h"
 #include "me_device.hpp"
 
 namespace me {
 
-struct MlpState {
-    int nin = 0;
-    int trained = 0;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int MLP_H = 256;      // hidden width
+constexpr int MLP_OUT = 10;     // 9 parameters with phi as (sin 2phi, cos 2phi)
+constexpr int MLP_OUTP = 16;    // output width padded for the MFMA tile
+constexpr int MLP_NL = 4;       // layers
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return __float2bfloat16(x); }
+__device__ __forceinline__ float from_bf16(__bf16 x) { return __bfloat162float(x); }
+
+// GELU(z) = z Phi(z) and its derivative Phi(z) + z phi(z)
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
+
+// C[M][N] = A[M][K] * B[K][N] on bf16 operands, FP32 accumulation.
+// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]: transposes are strides.
+// 256 threads compute a 64x64 tile; each wave owns 16 rows and 4 MFMA
+// 16x16 tiles; K advances 32 at a time through LDS (M, N and K are
+// bounds-checked; out-of-range operands load as zero).  Epilogues:
+//   EPI_FWD_GELU    Z = C + bias (fp32 out), Y = gelu(Z) (bf16 out)
+//   EPI_FWD_LINEAR  Z = C + bias (fp32 out)
+//   EPI_BWD_GELU    Y = C * gelu'(aux) (bf16 out; aux = pre-activation fp32)
+//   EPI_F32         Z = C * scale (fp32 out)
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
+                                                 const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
+                                                 int K, float *__restrict__ Zf, __bf16 *__restrict__ Yb, int ldc,
+                                                 const float *__restrict__ bias, const float *__restrict__ aux,
+                                                 float scale)
+{
+    __shared__ __bf16 sA[64][32 + 8];
+    __shared__ __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        // 64x32 tiles of A and B^T: 2048 elements each, 8 per thread
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = tid * 8 + e;
+            const int r = idx >> 5, kk = idx & 31;
+            const int m = m0 + r, n = n0 + r, k = k0 + kk;
+            sA[r][kk] = (m < M && k < K) ? A[(int64_t)m * sam + (int64_t)k * sak] : to_bf16(0.f);
+            sB[r][kk] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : to_bf16(0.f);
+        }
+        __syncthreads();
+        // lane l: row/col (l % 16), k group 8*(l / 16) .. +7
+        const int rr = lane & 15, kg = (lane >> 4) * 8;
+        bf16x8 a;
+#pragma unroll
+        for (int q = 0; q < 8; q++) a[q] = sA[wave * 16 + rr][kg + q];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            bf16x8 b;
+#pragma unroll
+            for (int q = 0; q < 8; q++) b[q] = sB[t * 16 + rr][kg + q];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout: lane l holds column (l % 16), rows 4*(l/16) .. +3
+    const int col = lane & 15, rb = (lane >> 4) * 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int n = n0 + t * 16 + col;
+        if (n >= N) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int m = m0 + wave * 16 + rb + v;
+            if (m >= M) continue;
+            const float c = acc[t][v];
+            const int64_t o = (int64_t)m * ldc + n;
+            if (EPI == EPI_FWD_GELU) {
+                const float z = c + bias[n];
+                Zf[o] = z;
+                Yb[o] = to_bf16(gelu(z));
+            } else if (EPI == EPI_FWD_LINEAR) {
+                Zf[o] = c + bias[n];
+            } else if (EPI == EPI_BWD_GELU) {
+                Yb[o] = to_bf16(c * gelu_grad(aux[o]));
+            } else {
+                Zf[o] = c * scale;
+            }
+        }
+    }
+}
+
+// profiles -> normalised bf16 inputs [n][nin_pad]
+template <typename TIn>
+__global__ void mlp_inputs(const TIn *__restrict__ prof, int64_t n, int nl, int nin_pad, __bf16 *__restrict__ X)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const TIn *pr = prof + p * 4 * nl;
+    float imax = 0.f;
+    for (int i = 0; i < nl; i++) imax = fmaxf(imax, (float)pr[i]);
+    const float inv = imax > 0.f ? 1.0f / imax : 0.f;
+    for (int i = 0; i < nin_pad; i++) {
+        float x = 0.f;
+        if (i < nl) x = (float)pr[i] * inv - 1.0f;
+        else if (i < 4 * nl) x = 10.0f * (float)pr[i] * inv;
+        if (!isfinite(x)) x = 0.f;
+        X[p * nin_pad + i] = to_bf16(x);
+    }
+}
+
+struct MlpNorm {
+    double lo[9], hi[9];
 };
 
-inline void mlp_init(MlpState &m, int nin) { m.nin = nin; m.trained = 0; }
-inline void mlp_free(MlpState &m) { m.trained = 0; }
-inline int64_t mlp_num_params(const MlpState &) { return 0; }
-inline me_status mlp_get_params(MlpState &, hipStream_t, float *) { return ME_ENOTREADY; }
-inline me_status mlp_set_params(MlpState &, hipStream_t, const float *) { return ME_ENOTREADY; }
-inline me_status mlp_grad_host(MlpState &, hipStream_t, const float *, const float *, int, float *, double *)
+// truths -> normalised targets [n][16] (10 used)
+__global__ void mlp_targets(const double *__restrict__ truth, const float *__restrict__ prof, int64_t n, int nl,
+                            MlpNorm nm, float *__restrict__ Y)
 {
-    return ME_ENOTREADY;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double *t = truth + p * 9;
+    float imax = 0.f;
+    for (int i = 0; i < nl; i++) imax = fmaxf(imax, prof[p * 4 * nl + i]);
+    float *y = Y + p * MLP_OUTP;
+    int o = 0;
+    for (int k = 0; k < 9; k++) {
+        if (k == 2) {
+            y[o++] = (float)sin(2.0 * t[2]);
+            y[o++] = (float)cos(2.0 * t[2]);
+        } else if (k == 7) {
+            y[o++] = imax > 0.f ? (float)(t[7] / imax) : 0.f;
+        } else {
+            y[o++] = (float)((t[k] - nm.lo[k]) / (nm.hi[k] - nm.lo[k]));
+        }
+    }
+    for (; o < MLP_OUTP; o++) y[o] = 0.f;
 }
-inline me_status mlp_train(MlpState &, const DevCfg &, hipStream_t, uint64_t, const double *, double, double, double,
-                           int64_t, int, int, double, double *)
+
+// network outputs -> starting points [n][9] (double)
+template <typename TIn>
+__global__ void mlp_decode(const float *__restrict__ Z, const TIn *__restrict__ prof, int64_t n, int nl, MlpNorm nm,
+                           double *__restrict__ init)
 {
-    return ME_ENOTREADY;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float *z = Z + p * MLP_OUTP;
+    float imax = 0.f;
+    for (int i = 0; i < nl; i++) imax = fmaxf(imax, (float)prof[p * 4 * nl + i]);
+    double *o = init + p * 9;
+    int j = 0;
+    for (int k = 0; k < 9; k++) {
+        if (k == 2) {
+            double s = z[j++], c = z[j++];
+            double ph = 0.5 * atan2(s, c);
+            if (ph < 0) ph += kPi;
+            o[2] = ph;
+        } else if (k == 7) {
+            o[7] = (double)z[j++] * imax;
+        } else {
+            double u = z[j++];
+            u = u < 0 ? 0 : (u > 1 ? 1 : u);
+            o[k] = nm.lo[k] + u * (nm.hi[k] - nm.lo[k]);
+        }
+    }
 }
-inline me_status mlp_predict(MlpState &, const DevCfg &, hipStream_t, const float *, int, int64_t, double *)
+
+// loss = mean over rows and the 10 outputs of (Z - Y)^2; dZ = 2 (Z - Y) / (n*10)
+__global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ Y, int n, __bf16 *__restrict__ dZ,
+                         float *__restrict__ loss_acc)
 {
-    return ME_ENOTREADY;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (p < n) {
+        for (int o = 0; o < MLP_OUTP; o++) {
+            float d = 0.f;
+            if (o < MLP_OUT) {
+                d = Z[p * MLP_OUTP + o] - Y[p * MLP_OUTP + o];
+                l += d * d;
+            }
+            dZ[p * MLP_OUTP + o] = to_bf16(2.0f * d / (float)(n * MLP_OUT));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss_acc, l / (float)(n * MLP_OUT));
+}
+
+// column sums of a bf16 [n][w] matrix (bias gradients)
+__global__ void col_sum(const __bf16 *__restrict__ D, int n, int w, float *__restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w) return;
+    float s = 0.f;
+    for (int r = 0; r < n; r++) s += from_bf16(D[(int64_t)r * w + c]);
+    out[c] = s;
+}
+
+// Adam on FP32 masters, then the bf16 copy used by the GEMMs
+__global__ void adam_step(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                          const float *__restrict__ g, __bf16 *__restrict__ pb, int64_t n, float lr, float b1, float b2,
+                          float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (pb) pb[i] = to_bf16(pi);
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// ---------------------------------------------------------------- host side
+struct MlpState {
+    int nin = 0, ninp = 0, trained = 0;
+    int dims[MLP_NL + 1] = {0, MLP_H, MLP_H, MLP_H, MLP_OUTP};
+    // parameters: for each layer W [din][dout] then b [dout], FP32 masters
+    float *P = nullptr, *Mo = nullptr, *Vo = nullptr, *G = nullptr;
+    __bf16 *Pb = nullptr;
+    int64_t np = 0;
+    int64_t offW[MLP_NL] = {0}, offB[MLP_NL] = {0};
+    MlpNorm norm;
+    // activations for up to `cap` rows
+    int64_t cap = 0;
+    __bf16 *X = nullptr, *A[MLP_NL] = {nullptr}, *D[MLP_NL] = {nullptr};
+    float *Z[MLP_NL] = {nullptr}, *Y = nullptr, *loss = nullptr;
+    float *prof_tmp = nullptr;
+    double *truth_tmp = nullptr;
+    int64_t tmp_cap = 0;
+};
+
+inline int64_t mlp_num_params(const MlpState &m)
+{
+    // exported layout: only the real input rows (nin) and output columns (10)
+    if (m.nin == 0) return 0;
+    return (int64_t)m.nin * MLP_H + MLP_H + 2 * ((int64_t)MLP_H * MLP_H + MLP_H) + (int64_t)MLP_H * MLP_OUT + MLP_OUT;
+}
+
+inline void mlp_free(MlpState &m)
+{
+    hipFree(m.P); hipFree(m.Mo); hipFree(m.Vo); hipFree(m.G); hipFree(m.Pb);
+    hipFree(m.X); hipFree(m.Y); hipFree(m.loss); hipFree(m.prof_tmp); hipFree(m.truth_tmp);
+    for (int l = 0; l < MLP_NL; l++) { hipFree(m.A[l]); hipFree(m.D[l]); hipFree(m.Z[l]); }
+    m = MlpState();
+}
+
+inline void mlp_init(MlpState &m, int nin)
+{
+    m.nin = nin;
+    m.ninp = (nin + 31) / 32 * 32;
+    m.dims[0] = m.ninp;
+}
+
+inline me_status mlp_alloc_params(MlpState &m)
+{
+    if (m.P) return ME_OK;
+    int64_t off = 0;
+    for (int l = 0; l < MLP_NL; l++) {
+        m.offW[l] = off;
+        off += (int64_t)m.dims[l] * m.dims[l + 1];
+        m.offB[l] = off;
+        off += m.dims[l + 1];
+    }
+    m.np = off;
+    hipError_t e = hipMalloc(&m.P, sizeof(float) * off);
+    e = e ? e : hipMalloc(&m.Mo, sizeof(float) * off);
+    e = e ? e : hipMalloc(&m.Vo, sizeof(float) * off);
+    e = e ? e : hipMalloc(&m.G, sizeof(float) * off);
+    e = e ? e : hipMalloc(&m.Pb, sizeof(__bf16) * off);
+    e = e ? e : hipMalloc(&m.loss, sizeof(float));
+    return e == hipSuccess ? ME_OK : ME_ENOMEM;
+}
+
+inline me_status mlp_alloc_acts(MlpState &m, int64_t rows)
+{
+    if (rows <= m.cap) return ME_OK;
+    hipFree(m.X); hipFree(m.Y);
+    for (int l = 0; l < MLP_NL; l++) { hipFree(m.A[l]); hipFree(m.D[l]); hipFree(m.Z[l]); }
+    hipError_t e = hipMalloc(&m.X, sizeof(__bf16) * rows * m.ninp);
+    e = e ? e : hipMalloc(&m.Y, sizeof(float) * rows * MLP_OUTP);
+    for (int l = 0; l < MLP_NL; l++) {
+        const int w = m.dims[l + 1];
+        e = e ? e : hipMalloc(&m.A[l], sizeof(__bf16) * rows * w);
+        e = e ? e : hipMalloc(&m.D[l], sizeof(__bf16) * rows * w);
+        e = e ? e : hipMalloc(&m.Z[l], sizeof(float) * rows * w);
+    }
+    if (e != hipSuccess) return ME_ENOMEM;
+    m.cap = rows;
+    return ME_OK;
+}
+
+// He-style uniform initialisation from a host LCG (deterministic)
+inline me_status mlp_init_params(MlpState &m, hipStream_t s, uint64_t seed)
+{
+    std::vector<float> h(m.np, 0.f);
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return ((st >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (int l = 0; l < MLP_NL; l++) {
+        const int din = m.dims[l], dout = m.dims[l + 1];
+        const int rin = l == 0 ? m.nin : din, rout = l == MLP_NL - 1 ? MLP_OUT : dout;
+        const double a = sqrt(6.0 / rin);
+        for (int i = 0; i < rin; i++)
+            for (int j = 0; j < rout; j++) h[m.offW[l] + (int64_t)i * dout + j] = (float)(a * rnd() * (l == MLP_NL - 1 ? 0.5 : 1.0));
+    }
+    hipError_t e = hipMemcpyAsync(m.P, h.data(), sizeof(float) * m.np, hipMemcpyHostToDevice, s);
+    e = e ? e : hipMemsetAsync(m.Mo, 0, sizeof(float) * m.np, s);
+    e = e ? e : hipMemsetAsync(m.Vo, 0, sizeof(float) * m.np, s);
+    if (e != hipSuccess) return ME_EDEVICE;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Pb, m.np);
+    return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
+}
+
+inline dim3 gemm_grid(int M, int N) { return dim3((M + 63) / 64, (N + 63) / 64); }
+
+// forward pass over `rows` rows of m.X
+inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
+{
+    const __bf16 *in = m.X;
+    for (int l = 0; l < MLP_NL; l++) {
+        const int din = m.dims[l], dout = m.dims[l + 1];
+        const __bf16 *W = m.Pb + m.offW[l];
+        const float *b = m.P + m.offB[l];
+        if (l < MLP_NL - 1)
+            hipLaunchKernelGGL(gemm_bf16<EPI_FWD_GELU>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
+                               (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], m.A[l], dout, b,
+                               (const float *)nullptr, 1.0f);
+        else
+            hipLaunchKernelGGL(gemm_bf16<EPI_FWD_LINEAR>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
+                               (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], (__bf16 *)nullptr,
+                               dout, b, (const float *)nullptr, 1.0f);
+        in = m.A[l];
+    }
+}
+
+// backward pass (loss gradient already in m.D[last]); gradients into m.G
+inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
+{
+    for (int l = MLP_NL - 1; l >= 0; l--) {
+        const int din = m.dims[l], dout = m.dims[l + 1];
+        const __bf16 *act = l == 0 ? m.X : m.A[l - 1];
+        // dW = act^T dZ  (M = din, N = dout, K = rows)
+        hipLaunchKernelGGL(gemm_bf16<EPI_F32>, gemm_grid(din, dout), dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
+                           (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.G + m.offW[l],
+                           (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f);
+        hipLaunchKernelGGL(col_sum, dim3((dout + 255) / 256), dim3(256), 0, s, (const __bf16 *)m.D[l], rows, dout,
+                           m.G + m.offB[l]);
+        if (l > 0) {
+            // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
+            hipLaunchKernelGGL(gemm_bf16<EPI_BWD_GELU>, gemm_grid(rows, din), dim3(256), 0, s,
+                               (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, (const __bf16 *)(m.Pb + m.offW[l]),
+                               (int64_t)1, (int64_t)dout, rows, din, dout, (float *)nullptr, m.D[l - 1], din,
+                               (const float *)nullptr, (const float *)m.Z[l - 1], 1.0f);
+        }
+    }
+}
+
+inline me_status mlp_tmp(MlpState &m, int64_t rows, int nl)
+{
+    if (rows <= m.tmp_cap) return ME_OK;
+    hipFree(m.prof_tmp);
+    hipFree(m.truth_tmp);
+    hipError_t e = hipMalloc(&m.prof_tmp, sizeof(float) * rows * 4 * nl);
+    e = e ? e : hipMalloc(&m.truth_tmp, sizeof(double) * rows * 9);
+    if (e != hipSuccess) return ME_ENOMEM;
+    m.tmp_cap = rows;
+    return ME_OK;
+}
+
+// Training: the dataset (n_train profiles + truths) is synthesised once by
+// gen (the device map generator); epochs sweep it in batches.
+template <typename GenFn>
+me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t seed, const double *ranges,
+                         int64_t n_train, int batch, int epochs, double lr, double *final_loss, GenFn &&gen)
+{
+    batch = (batch + 63) / 64 * 64;
+    if (batch > n_train) batch = (int)((n_train / 64) * 64);
+    if (batch < 64) return ME_EINVAL;
+    const int nl = C.nl;
+    me_status st = mlp_alloc_params(m);
+    if (st != ME_OK) return st;
+    if ((st = mlp_alloc_acts(m, batch)) != ME_OK) return st;
+    if ((st = mlp_tmp(m, n_train, nl)) != ME_OK) return st;
+    for (int k = 0; k < 9; k++) {
+        m.norm.lo[k] = ranges[2 * k];
+        m.norm.hi[k] = ranges[2 * k + 1] > ranges[2 * k] ? ranges[2 * k + 1] : ranges[2 * k] + 1.0;
+    }
+    if ((st = mlp_init_params(m, s, seed)) != ME_OK) return st;
+    if ((st = gen(m.prof_tmp, m.truth_tmp, n_train)) != ME_OK) return st;
+    const int64_t nb = n_train / batch;
+    const int64_t total = nb * epochs;
+    float b1 = 0.9f, b2 = 0.999f;
+    double lossv = 0;
+    for (int64_t step = 0; step < total; step++) {
+        const int64_t r0 = (step % nb) * batch;
+        hipLaunchKernelGGL(mlp_inputs<float>, dim3((batch + 255) / 256), dim3(256), 0, s,
+                           (const float *)(m.prof_tmp + r0 * 4 * nl), (int64_t)batch, nl, m.ninp, m.X);
+        hipLaunchKernelGGL(mlp_targets, dim3((batch + 255) / 256), dim3(256), 0, s,
+                           (const double *)(m.truth_tmp + r0 * 9), (const float *)(m.prof_tmp + r0 * 4 * nl),
+                           (int64_t)batch, nl, m.norm, m.Y);
+        mlp_forward(m, s, batch);
+        hipMemsetAsync(m.loss, 0, sizeof(float), s);
+        hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
+                           (const float *)m.Y, batch, m.D[MLP_NL - 1], m.loss);
+        mlp_backward(m, s, batch);
+        // cosine decay to 5% of lr
+        const double frac = (double)step / (double)total;
+        const float lrs = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(kPi * frac))));
+        const float c1 = 1.f - powf(b1, (float)(step + 1)), c2 = 1.f - powf(b2, (float)(step + 1));
+        hipLaunchKernelGGL(adam_step, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Mo, m.Vo, (const float *)m.G,
+                           m.Pb, m.np, lrs, b1, b2, 1e-8f, c1, c2);
+        if (step == total - 1) {
+            float lh = 0;
+            hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
+            hipStreamSynchronize(s);
+            lossv = lh;
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) return ME_EDEVICE;
+    if (final_loss) *final_loss = lossv;
+    m.trained = 1;
+    return ME_OK;
+}
+
+// predictions for n device profiles, processed in chunks of the activation capacity
+inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const void *prof, int prof_is_f64, int64_t n,
+                             double *init)
+{
+    if (!m.trained) return ME_ENOTREADY;
+    const int nl = C.nl;
+    const int64_t chunk = 1 << 20;
+    me_status st = mlp_alloc_acts(m, n < chunk ? ((n + 63) / 64) * 64 : chunk);
+    if (st != ME_OK) return st;
+    for (int64_t r0 = 0; r0 < n; r0 += m.cap) {
+        const int rows = (int)((n - r0) < m.cap ? (n - r0) : m.cap);
+        if (prof_is_f64)
+            hipLaunchKernelGGL(mlp_inputs<double>, dim3((rows + 255) / 256), dim3(256), 0, s,
+                               (const double *)prof + r0 * 4 * nl, (int64_t)rows, nl, m.ninp, m.X);
+        else
+            hipLaunchKernelGGL(mlp_inputs<float>, dim3((rows + 255) / 256), dim3(256), 0, s,
+                               (const float *)prof + r0 * 4 * nl, (int64_t)rows, nl, m.ninp, m.X);
+        mlp_forward(m, s, rows);
+        if (prof_is_f64)
+            hipLaunchKernelGGL(mlp_decode<double>, dim3((rows + 255) / 256), dim3(256), 0, s,
+                               (const float *)m.Z[MLP_NL - 1], (const double *)prof + r0 * 4 * nl, (int64_t)rows, nl,
+                               m.norm, init + r0 * 9);
+        else
+            hipLaunchKernelGGL(mlp_decode<float>, dim3((rows + 255) / 256), dim3(256), 0, s,
+                               (const float *)m.Z[MLP_NL - 1], (const float *)prof + r0 * 4 * nl, (int64_t)rows, nl,
+                               m.norm, init + r0 * 9);
+    }
+    return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
+}
+
+// exported parameter order: per layer W (real rows x real cols, row-major) then b (real cols)
+template <typename F> inline void mlp_for_each_exported(const MlpState &m, F &&f)
+{
+    int64_t e = 0;
+    for (int l = 0; l < MLP_NL; l++) {
+        const int din = m.dims[l], dout = m.dims[l + 1];
+        const int rin = l == 0 ? m.nin : din, rout = l == MLP_NL - 1 ? MLP_OUT : dout;
+        for (int i = 0; i < rin; i++)
+            for (int j = 0; j < rout; j++) f(e++, m.offW[l] + (int64_t)i * dout + j);
+        for (int j = 0; j < rout; j++) f(e++, m.offB[l] + j);
+    }
+}
+
+inline me_status mlp_get_params(MlpState &m, hipStream_t s, float *out)
+{
+    if (!m.P) return ME_ENOTREADY;
+    std::vector<float> h(m.np);
+    if (hipMemcpyAsync(h.data(), m.P, sizeof(float) * m.np, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ME_EDEVICE;
+    mlp_for_each_exported(m, [&](int64_t e, int64_t i) { out[e] = h[i]; });
+    return ME_OK;
+}
+
+inline me_status mlp_set_params(MlpState &m, hipStream_t s, const float *in)
+{
+    me_status st = mlp_alloc_params(m);
+    if (st != ME_OK) return st;
+    std::vector<float> h(m.np, 0.f);
+    mlp_for_each_exported(m, [&](int64_t e, int64_t i) { h[i] = in[e]; });
+    if (hipMemcpyAsync(m.P, h.data(), sizeof(float) * m.np, hipMemcpyHostToDevice, s) != hipSuccess) return ME_EDEVICE;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Pb, m.np);
+    if (hipStreamSynchronize(s) != hipSuccess) return ME_EDEVICE;
+    return ME_OK;
+}
+
+// one forward + backward pass on a host batch (inputs already normalised,
+// [batch][4*nl]; targets [batch][10]); gradient in the exported order
+inline me_status mlp_grad_host(MlpState &m, hipStream_t s, const float *x, const float *y, int batch, float *grad,
+                               double *loss)
+{
+    if (!m.P) return ME_ENOTREADY;
+    if (batch % 32) return ME_EINVAL;  // batch is the K of the weight-gradient GEMM
+    me_status st = mlp_alloc_acts(m, batch);
+    if (st != ME_OK) return st;
+    std::vector<__bf16> hx((size_t)batch * m.ninp);
+    std::vector<float> hy((size_t)batch * MLP_OUTP, 0.f);
+    for (int r = 0; r < batch; r++) {
+        for (int i = 0; i < m.ninp; i++) hx[(size_t)r * m.ninp + i] = __float2bfloat16(i < m.nin ? x[(size_t)r * m.nin + i] : 0.f);
+        for (int o = 0; o < MLP_OUT; o++) hy[(size_t)r * MLP_OUTP + o] = y[(size_t)r * MLP_OUT + o];
+    }
+    hipError_t e = hipMemcpyAsync(m.X, hx.data(), sizeof(__bf16) * hx.size(), hipMemcpyHostToDevice, s);
+    e = e ? e : hipMemcpyAsync(m.Y, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, s);
+    e = e ? e : hipMemsetAsync(m.loss, 0, sizeof(float), s);
+    if (e != hipSuccess) return ME_EDEVICE;
+    mlp_forward(m, s, batch);
+    hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
+                       (const float *)m.Y, batch, m.D[MLP_NL - 1], m.loss);
+    mlp_backward(m, s, batch);
+    std::vector<float> hg(m.np);
+    float lh = 0;
+    e = hipMemcpyAsync(hg.data(), m.G, sizeof(float) * m.np, hipMemcpyDeviceToHost, s);
+    e = e ? e : hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
+    e = e ? e : hipStreamSynchronize(s);
+    if (e != hipSuccess) return ME_EDEVICE;
+    mlp_for_each_exported(m, [&](int64_t ei, int64_t i) { grad[ei] = hg[i]; });
+    if (loss) *loss = lh;
+    return ME_OK;
 }
 
 }  // namespace me
