@@ -47,7 +47,7 @@ enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
 //   EPI_FWD_GELU    Z = C + bias (fp32 out), Y = gelu(Z) (bf16 out)
 //   EPI_FWD_LINEAR  Z = C + bias (fp32 out)
 //   EPI_BWD_GELU    Y = C * gelu'(aux) (bf16 out; aux = pre-activation fp32)
-//   EPI_F32         Z = C * scale (fp32 out)
+//   EPI_F32         Z += C * scale (fp32 atomics; K split over gridDim.z)
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
                                                  const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
@@ -59,18 +59,22 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
     __shared__ __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    // split K over gridDim.z (weight gradients: K is the batch)
+    const int kchunk = ((K + 32 * gridDim.z - 1) / (32 * gridDim.z)) * 32;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = K < kbeg + kchunk ? K : kbeg + kchunk;
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
         // 64x32 tiles of A and B^T: 2048 elements each, 8 per thread
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int idx = tid * 8 + e;
             const int r = idx >> 5, kk = idx & 31;
             const int m = m0 + r, n = n0 + r, k = k0 + kk;
-            sA[r][kk] = (m < M && k < K) ? A[(int64_t)m * sam + (int64_t)k * sak] : to_bf16(0.f);
-            sB[r][kk] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : to_bf16(0.f);
+            sA[r][kk] = (m < M && k < kend) ? A[(int64_t)m * sam + (int64_t)k * sak] : to_bf16(0.f);
+            sB[r][kk] = (n < N && k < kend) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : to_bf16(0.f);
         }
         __syncthreads();
         // lane l: row/col (l % 16), k group 8*(l / 16) .. +7
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
             } else if (EPI == EPI_BWD_GELU) {
                 Yb[o] = to_bf16(c * gelu_grad(aux[o]));
             } else {
-                Zf[o] = c * scale;
+                atomicAdd(&Zf[o], c * scale);  // split-K partial sums (Zf zeroed by the caller)
             }
         }
     }
@@ -209,14 +213,24 @@ __global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ 
     if ((threadIdx.x & 63) == 0) atomicAdd(loss_acc, l / (float)(n * MLP_OUT));
 }
 
-// column sums of a bf16 [n][w] matrix (bias gradients)
-__global__ void col_sum(const __bf16 *__restrict__ D, int n, int w, float *__restrict__ out)
+// column sums of a bf16 [n][w] matrix (bias gradients): 256 threads = 64
+// columns x 4 row groups per block, blockIdx.y selects a chunk of rows;
+// partial sums go to out with atomics (out zeroed by the caller)
+__global__ __launch_bounds__(256) void col_sum(const __bf16 *__restrict__ D, int n, int w, int rows_per_chunk,
+                                               float *__restrict__ out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w) return;
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int g = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * rows_per_chunk;
+    const int r1 = n < r0 + rows_per_chunk ? n : r0 + rows_per_chunk;
+    __shared__ float part[4][64];
     float s = 0.f;
-    for (int r = 0; r < n; r++) s += from_bf16(D[(int64_t)r * w + c]);
-    out[c] = s;
+    if (c < w)
+        for (int r = r0 + g; r < r1; r += 4) s += from_bf16(D[(int64_t)r * w + c]);
+    part[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && c < w) atomicAdd(&out[c], part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                              part[3][threadIdx.x]);
 }
 
 // Adam on FP32 masters, then the bf16 copy used by the GEMMs
@@ -234,6 +248,12 @@ __global__ void adam_step(float *__restrict__ p, float *__restrict__ m, float *_
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
     if (pb) pb[i] = to_bf16(pi);
+}
+
+__global__ void fill_f32(float *__restrict__ x, int64_t n, float v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__ y, int64_t n)
@@ -370,15 +390,21 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
 // backward pass (loss gradient already in m.D[last]); gradients into m.G
 inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
 {
+    // gradients accumulate with atomics: clear them with a fill kernel
+    hipLaunchKernelGGL(fill_f32, dim3((m.np + 255) / 256), dim3(256), 0, s, m.G, m.np, 0.0f);
+    const int ksplit = rows >= 4096 ? 32 : (rows >= 1024 ? 8 : 1);
+    const int rchunk = (rows + 63) / 64;
     for (int l = MLP_NL - 1; l >= 0; l--) {
         const int din = m.dims[l], dout = m.dims[l + 1];
         const __bf16 *act = l == 0 ? m.X : m.A[l - 1];
-        // dW = act^T dZ  (M = din, N = dout, K = rows)
-        hipLaunchKernelGGL(gemm_bf16<EPI_F32>, gemm_grid(din, dout), dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
+        // dW = act^T dZ  (M = din, N = dout, K = rows split over gridDim.z)
+        dim3 g = gemm_grid(din, dout);
+        g.z = ksplit;
+        hipLaunchKernelGGL(gemm_bf16<EPI_F32>, g, dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
                            (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.G + m.offW[l],
                            (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f);
-        hipLaunchKernelGGL(col_sum, dim3((dout + 255) / 256), dim3(256), 0, s, (const __bf16 *)m.D[l], rows, dout,
-                           m.G + m.offB[l]);
+        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, (rows + rchunk - 1) / rchunk), dim3(256), 0, s,
+                           (const __bf16 *)m.D[l], rows, dout, rchunk, m.G + m.offB[l]);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
             hipLaunchKernelGGL(gemm_bf16<EPI_BWD_GELU>, gemm_grid(rows, din), dim3(256), 0, s,
@@ -433,7 +459,7 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
                            (const double *)(m.truth_tmp + r0 * 9), (const float *)(m.prof_tmp + r0 * 4 * nl),
                            (int64_t)batch, nl, m.norm, m.Y);
         mlp_forward(m, s, batch);
-        hipMemsetAsync(m.loss, 0, sizeof(float), s);
+        hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, s, m.loss, (int64_t)1, 0.0f);
         hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
                            (const float *)m.Y, batch, m.D[MLP_NL - 1], m.loss);
         mlp_backward(m, s, batch);
