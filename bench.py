@@ -42,16 +42,41 @@ def op_model(st, nl: int, fp64: bool):
     return float(f_core + f_fadd), float(f_svd)
 
 
-def roofline(st, nl, fp64, kernel_s):
+def roofline(st, nl, fp64, kernel_s, traffic=None):
     f32, f64 = op_model(st, nl, fp64)
     t_min = f32 / P32 + f64 / P64
     achieved = (f32 + f64) / kernel_s / 1e12
     peak = (f32 + f64) / t_min / 1e12 if t_min > 0 else 0.0
     return {"bound": "compute", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4) if peak else 0.0, "traffic": None,
+            "frac": round(achieved / peak, 4) if peak else 0.0, "traffic": traffic,
             "flops_fp32": f32, "flops_fp64": f64,
             "note": "vector-ALU bound: FP32 synthesis/Faddeeva/Jacobian/normal matrix + FP64 9x9 SVD; "
                     "peak = (F32+F64)/(F32/157.3T + F64/78.6T)"}
+
+
+def hbm_traffic(workload: str, mode: str, precision: str):
+    """HBM bytes per inversion launch, read from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (already scaled by the plain-copy calibration)."""
+    p = REPO / "profiles" / "invert_kernel_hbm_traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        f = line.split()
+        if len(f) >= 5 and f[:3] == [workload, mode, precision]:
+            return float(f[3]) + float(f[4])
+    return None
+
+
+def copy_probe(mib: int):
+    """A plain device copy of a known size: what FETCH_SIZE / WRITE_SIZE read
+    for it calibrates the counters (a copy reads and writes `mib` MiB)."""
+    import torch
+    a = torch.ones(mib * 2**18, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy probe: 3 copies of {mib} MiB", flush=True)
 
 
 def cpu_baseline(w, cfg, target_s: float, threads: int):
@@ -89,7 +114,12 @@ def main():
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(pixels are independent: a multi-GPU run would give each GPU a row slab)", file=sys.stderr)
@@ -189,7 +219,7 @@ def main():
                                + (f", {int(w.ff_frac * 100)}% field-free" if w.ff_frac else ""),
                    "pixels": n, "init": "learned MLP + LM, fallback plain" if headline == "learned" else "heuristic",
                    "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision},
-        "roofline": roofline(last, w.nl, fp64, kernel_s),
+        "roofline": roofline(last, w.nl, fp64, kernel_s, hbm_traffic(w.name, headline, args.precision)),
         "cpu_baseline": cpu,
         "details": dict(results, median_chi2=float(np.median(res[:, 9])),
                         speedup_learned_vs_plain=(round(results["learned"]["converged_pixels_per_s"]

@@ -718,6 +718,19 @@ __global__ __launch_bounds__(256) void fit_kernel(DevCfg C, const double *fitpar
 }
 
 // synthetic map: Philox truths, FP64 synthesis, Gaussian noise sigma*S
+// Truth draw and noisy sample with separate roundings (no FMA contraction), so
+// the map is bit-identical to the one the host forward model builds.
+__device__ __forceinline__ double lerp_unfused(double lo, double hi, double u)
+{
+#pragma clang fp contract(off)
+    return lo + (hi - lo) * u;
+}
+__device__ __forceinline__ float noisy_unfused(double o, double sigma, double scale, double g)
+{
+#pragma clang fp contract(off)
+    return (float)(o + sigma * scale * g);
+}
+
 __global__ __launch_bounds__(256) void genmap_kernel(DevCfg C, uint64_t seed, const double *ranges, double ff_frac,
                                                      double ff_bmax, double sigma, int64_t n, float *prof,
                                                      double *truth)
@@ -735,7 +748,7 @@ __global__ __launch_bounds__(256) void genmap_kernel(DevCfg C, uint64_t seed, co
     }
     double t[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) t[k] = ranges[2 * k] + (ranges[2 * k + 1] - ranges[2 * k]) * u01(w[k]);
+    for (int k = 0; k < 9; k++) t[k] = lerp_unfused(ranges[2 * k], ranges[2 * k + 1], u01(w[k]));
     if (ff_frac > 0 && u01(w[9]) < ff_frac) t[0] = ff_bmax * u01(w[0]);
     PixK<double> K;
     pix_consts<double>(C, t, K);
@@ -745,7 +758,7 @@ __global__ __launch_bounds__(256) void genmap_kernel(DevCfg C, uint64_t seed, co
         gauss4(seed, p, (uint32_t)iw, kStreamNoise, gz);
         me_point<double, false>(C, K, 0.0, iw, o, NoSink{}, (FCount *)nullptr);
 #pragma unroll
-        for (int s = 0; s < 4; s++) prof[p * 4 * nl + s * nl + iw] = (float)(o[s] + sigma * t[7] * gz[s]);
+        for (int s = 0; s < 4; s++) prof[p * 4 * nl + s * nl + iw] = noisy_unfused(o[s], sigma, t[7], gz[s]);
     }
     if (truth)
 #pragma unroll
