@@ -83,11 +83,17 @@ def cpu_baseline(w, cfg, target_s: float, threads: int):
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle as O
     orc = O.Oracle(cfg, w.wavelengths())
-    n0 = 4 * threads
-    prof, _ = orc.gen_map(w.seed, w.ranges(), n0, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
-    t = time.perf_counter()
-    orc.invert(prof.astype(np.float64), nthreads=threads)
-    dt = time.perf_counter() - t
+    # size the sample: grow a calibration run until it takes ~1/10 of the
+    # target, then scale to the target
+    n0 = 16 * threads
+    while True:
+        prof, _ = orc.gen_map(w.seed, w.ranges(), n0, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
+        t = time.perf_counter()
+        orc.invert(prof.astype(np.float64), nthreads=threads)
+        dt = time.perf_counter() - t
+        if dt >= target_s / 10 or n0 >= w.n:
+            break
+        n0 = min(w.n, n0 * 4)
     n = int(min(w.n, max(n0, n0 * target_s / max(dt, 1e-3))))
     prof, _ = orc.gen_map(w.seed, w.ranges(), n, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
     t = time.perf_counter()

@@ -564,6 +564,44 @@ __device__ __forceinline__ double hyp(double a, double b)
     return mx * sqrt(1.0 + r * r);
 }
 
+// Reciprocal and reciprocal square root from the hardware estimates with two
+// Newton steps (each doubles the correct bits: ~1 ulp at the end).  The SVD
+// uses them in place of correctly rounded divisions and square roots; its
+// results move by rounding only.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_nr(double q)  // q normal, finite, > 0
+{
+    double r = __builtin_amdgcn_rsq(q);
+    double t = q * r;
+    double e = fma(-t, r, 1.0);
+    r = fma(0.5 * r, e, r);
+    t = q * r;
+    e = fma(-t, r, 1.0);
+    return fma(0.5 * r, e, r);
+}
+// z = hypot(f, h) and rz = 1 / z (0 when z = 0); squares stay in range on
+// the fast path, the scaled form covers the rest
+__device__ __forceinline__ void hyp_r(double f, double h, double &z, double &rz)
+{
+    const double af = fabs(f), ah = fabs(h);
+    const double mx = af > ah ? af : ah;
+    if (mx > 1e-140 && mx < 1e140) {
+        const double q = f * f + h * h;
+        rz = rsq_nr(q);
+        z = q * rz;
+    } else {
+        z = hyp(f, h);
+        rz = z != 0.0 ? 1.0 / z : 0.0;
+    }
+}
+
 // keeps the scheduler from hoisting work across SVD steps (live ranges of
 // the unrolled code would otherwise outgrow the register file)
 #define SVD_BARRIER() __builtin_amdgcn_sched_barrier(0)
@@ -611,21 +649,23 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
 #pragma unroll
         for (int k = i; k < 9; k++) scale += fabs(a[k * 9 + i]);
         if (scale != 0.0) {
+            const double rs = rcp_nr(scale);
 #pragma unroll
             for (int k = i; k < 9; k++) {
-                a[k * 9 + i] /= scale;
+                a[k * 9 + i] *= rs;
                 s += a[k * 9 + i] * a[k * 9 + i];
             }
             double f = a[i * 9 + i];
             g = f >= 0 ? -sqrt(s) : sqrt(s);
             double h = f * g - s;
             a[i * 9 + i] = f - g;
+            const double rh = rcp_nr(h);
             sfor_up<l, 9>([&](auto Jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(Jc)::value;
                 double d = 0.0;
 #pragma unroll
                 for (int k = i; k < 9; k++) d += a[k * 9 + i] * a[k * 9 + j];
-                double fc = d / h;
+                double fc = d * rh;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += fc * a[k * 9 + i];
                 SVD_BARRIER();
@@ -641,9 +681,10 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
 #pragma unroll
             for (int k = l; k < 9; k++) scale += fabs(a[i * 9 + k]);
             if (scale != 0.0) {
+                const double rs = rcp_nr(scale);
 #pragma unroll
                 for (int k = l; k < 9; k++) {
-                    a[i * 9 + k] /= scale;
+                    a[i * 9 + k] *= rs;
                     s += a[i * 9 + k] * a[i * 9 + k];
                 }
                 double f = a[i * 9 + l];
@@ -651,8 +692,9 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 double h = f * g - s;
                 a[i * 9 + l] = f - g;
                 double rv[9];
+                const double rh = rcp_nr(h);
 #pragma unroll
-                for (int k = l; k < 9; k++) rv[k] = a[i * 9 + k] / h;
+                for (int k = l; k < 9; k++) rv[k] = a[i * 9 + k] * rh;
                 sfor_up<l, 9>([&](auto Jr) __attribute__((always_inline)) {
                     constexpr int j = decltype(Jr)::value;
                     double d = 0.0;
@@ -678,12 +720,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         for (int j = l; j < 9; j++) a[i * 9 + j] = 0.0;
         if (gg != 0.0) {
             gg = 1.0 / gg;
+            const double rii = rcp_nr(a[i * 9 + i]);
             sfor_up<l, 9>([&](auto Jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(Jc)::value;
                 double d = 0.0;
 #pragma unroll
                 for (int k = l; k < 9; k++) d += a[k * 9 + i] * a[k * 9 + j];
-                double f = (d / a[i * 9 + i]) * gg;
+                double f = (d * rii) * gg;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += f * a[k * 9 + i];
                 SVD_BARRIER();
@@ -734,11 +777,11 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                             stop = true;
                         } else {
                             double gg = w[i];
-                            double h = hyp(f, gg);
+                            double h, rh;
+                            hyp_r(f, gg, h, rh);
                             w[i] = h;
-                            h = 1.0 / h;
-                            c = gg * h;
-                            s = -f * h;
+                            c = gg * rh;
+                            s = -f * rh;
 #pragma unroll
                             for (int r = 0; r < 9; r++) {
                                 double yv = col[r], zv = a[r * 9 + i];
@@ -785,20 +828,20 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                     yv = w[i];
                     h = s * gg;
                     gg *= c;
-                    z = hyp(f, h);
+                    double rz;
+                    hyp_r(f, h, z, rz);
                     e[j] = z;
-                    c = f / z;
-                    s = h / z;
+                    c = f * rz;
+                    s = h * rz;
                     f = x * c + gg * s;
                     gg = gg * c - x * s;
                     h = yv * s;
                     yv *= c;
-                    z = hyp(f, h);
+                    hyp_r(f, h, z, rz);
                     w[j] = z;
                     if (z != 0.0) {
-                        z = 1.0 / z;
-                        c = f * z;
-                        s = h * z;
+                        c = f * rz;
+                        s = h * rz;
                     }
                     f = c * gg + s * yv;
                     x = c * yv - s * gg;

@@ -317,6 +317,9 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
     double wmax = w[0];
 #pragma unroll
     for (int i = 1; i < 9; i++) wmax = wmax > w[i] ? wmax : w[i];
+    double rw[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) rw[j] = w[j] > 1e-50 ? rcp_nr(w[j]) : 0.0;
     double thr = thr0;
     r.nbk = 0;
     for (int pass = 0; pass < 64; pass++) {
@@ -325,7 +328,7 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
 #pragma unroll
         for (int j = 0; j < 9; j++) {
             if (w[j] < wt) w[j] = 0.;
-            y[j] = w[j] > 1e-50 ? (ug[j] / w[j]) / w[j] : 0.0;
+            y[j] = w[j] > 1e-50 ? (ug[j] * rw[j]) * rw[j] : 0.0;
         }
         double uy[9];
 #pragma unroll
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
         if (phase == PH_TRIAL && !ended) {
             // SVD of the input matrix (the normal matrix, or U of the previous
             // rejected trial: H1); U goes to the other slot and becomes the
-            // next trial's input
+            // next trial's input.  One call site for every lane of the wave.
             const Sol9 sv = svd_solve(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
                                       scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr);
             CT(CT_SVD) += 1;

@@ -30,11 +30,22 @@ constexpr int MLP_NL = 4;       // layers
 __device__ __forceinline__ __bf16 to_bf16(float x) { return __float2bfloat16(x); }
 __device__ __forceinline__ float from_bf16(__bf16 x) { return __bfloat162float(x); }
 
+// erf without branches: Abramowitz & Stegun 7.1.26 (Hastings), absolute
+// error below 1.5e-7 (activations are then rounded to bfloat16)
+__device__ __forceinline__ float erf_as(float x)
+{
+    const float ax = fabsf(x);
+    const float t = 1.0f / (1.0f + 0.3275911f * ax);
+    const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float y = 1.0f - poly * __expf(-ax * ax);
+    return copysignf(y, x);
+}
+
 // GELU(z) = z Phi(z) and its derivative Phi(z) + z phi(z)
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
 enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
@@ -262,6 +273,171 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__
     if (i < n) y[i] = to_bf16(x[i]);
 }
 
+// ------------------------------------------------ fused inference kernel
+// All four layers in one kernel: each wave keeps MI_C tiles of 16 pixels as
+// MFMA B operands (k x pixels) in registers, a layer's weights are staged in
+// LDS as A-operand fragments (out channels x k), and the D fragments of one
+// layer become the next layer's B fragments in place.  D gives a lane 4
+// consecutive output channels per 16-channel tile; the next layer reads its
+// k index in that order, so the packed weights of layers 1..3 permute their
+// input rows to match (mlp_in_channel) and no data moves between lanes.
+constexpr int MI_C = 2;                  // pixel tiles (16 each) per wave
+constexpr int MI_PX = 4 * 16 * MI_C;     // pixels per block pass
+
+// input channel that k element q of k-block kb, lane group g reads
+__device__ __forceinline__ int mlp_in_channel(int layer, int kb, int g, int q)
+{
+    return layer == 0 ? 32 * kb + 8 * g + q : 16 * (2 * kb + (q >> 2)) + 4 * g + (q & 3);
+}
+
+struct MlpPack {
+    int64_t offW[MLP_NL], offB[MLP_NL];   // masters (Pb / P)
+    int din[MLP_NL], dout[MLP_NL];
+    int T[MLP_NL], KB[MLP_NL];            // 16-row tiles and 32-wide k blocks
+    int64_t frag[MLP_NL];                 // first fragment of each layer in wpk
+};
+
+// one thread per (layer fragment, lane): 8 bf16 of the A fragment
+__global__ void mlp_pack_kernel(const __bf16 *__restrict__ Pb, const float *__restrict__ P, MlpPack d, int64_t total,
+                                bf16x8 *__restrict__ wpk, float *__restrict__ bpk)
+{
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < MLP_NL * 256) {
+        const int l = (int)(f / 256), o = (int)(f % 256);
+        bpk[f] = o < d.dout[l] ? P[d.offB[l] + o] : 0.f;
+    }
+    if (f >= total * 64) return;
+    const int64_t fr = f / 64;
+    const int lane = (int)(f % 64);
+    int l = MLP_NL - 1;
+    while (l > 0 && fr < d.frag[l]) l--;
+    const int64_t r = fr - d.frag[l];
+    const int t = (int)(r / d.KB[l]), kb = (int)(r % d.KB[l]);
+    const int o = 16 * t + (lane & 15), g = lane >> 4;
+    bf16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int i = mlp_in_channel(l, kb, g, q);
+        v[q] = (i < d.din[l] && o < d.dout[l]) ? Pb[d.offW[l] + (int64_t)i * d.dout[l] + o] : to_bf16(0.f);
+    }
+    wpk[f] = v;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const TIn *__restrict__ prof, int64_t n, int nl, int kb0,
+                                                        const bf16x8 *__restrict__ wpk, MlpPack d,
+                                                        const float *__restrict__ bpk, MlpNorm nm,
+                                                        double *__restrict__ init)
+{
+    __shared__ bf16x8 sW[16 * 8 * 64];   // 128 KiB: the fragments of one layer
+    __shared__ float sb[256];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, col = lane & 15, g = lane >> 4;
+    for (int64_t base = (int64_t)blockIdx.x * MI_PX; base < n; base += (int64_t)gridDim.x * MI_PX) {
+        bf16x8 act[MI_C][8];
+        float imax[MI_C];
+        // layer-0 operands: normalised profile values, k = input index
+#pragma unroll
+        for (int c = 0; c < MI_C; c++) {
+            const int64_t p = base + (wave * MI_C + c) * 16 + col;
+            const bool ok = p < n;
+            const TIn *pr = prof + (ok ? p : 0) * 4 * nl;
+            float im = 0.f;
+            if (ok)
+                for (int i = 0; i < nl; i++) im = fmaxf(im, (float)pr[i]);
+            imax[c] = im;
+            const float inv = im > 0.f ? 1.0f / im : 0.f;
+#pragma unroll
+            for (int kb = 0; kb < 8; kb++) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int i = 32 * kb + 8 * g + q;
+                    float x = 0.f;
+                    if (ok && kb < kb0) {
+                        if (i < nl) x = (float)pr[i] * inv - 1.0f;
+                        else if (i < 4 * nl) x = 10.0f * (float)pr[i] * inv;
+                        if (!isfinite(x)) x = 0.f;
+                    }
+                    act[c][kb][q] = to_bf16(x);
+                }
+            }
+        }
+#pragma unroll 1
+        for (int l = 0; l < MLP_NL; l++) {
+            const int KB = d.KB[l], T = d.T[l];
+            __syncthreads();
+            const bf16x8 *src = wpk + d.frag[l] * 64;
+            for (int i = tid; i < T * KB * 64; i += 256) sW[i] = src[i];
+            sb[tid] = bpk[l * 256 + tid];
+            __syncthreads();
+            if (l < MLP_NL - 1) {
+                bf16x8 nxt[MI_C][8];
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    // one output tile at a time: keeps the scheduler from
+                    // hoisting every tile's LDS reads into registers
+                    __builtin_amdgcn_sched_barrier(0);
+                    f32x4 acc[MI_C];
+#pragma unroll
+                    for (int c = 0; c < MI_C; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kb = 0; kb < 8; kb++) {
+                        if (kb < KB) {
+                            const bf16x8 a = sW[(t * KB + kb) * 64 + lane];
+#pragma unroll
+                            for (int c = 0; c < MI_C; c++)
+                                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, act[c][kb], acc[c], 0, 0, 0);
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < MI_C; c++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            nxt[c][t >> 1][4 * (t & 1) + v] = to_bf16(gelu(acc[c][v] + sb[16 * t + 4 * g + v]));
+                }
+#pragma unroll
+                for (int c = 0; c < MI_C; c++)
+#pragma unroll
+                    for (int kb = 0; kb < 8; kb++) act[c][kb] = nxt[c][kb];
+            } else {
+                // output layer: 16 channels (10 used); lane (g, col) holds
+                // channels 4g..4g+3 of pixel col; lanes of group 0 gather
+                // and decode
+#pragma unroll
+                for (int c = 0; c < MI_C; c++) {
+                    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kb = 0; kb < 8; kb++)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sW[kb * 64 + lane], act[c][kb], acc, 0, 0, 0);
+                    float z[16];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[v] += sb[4 * g + v];
+#pragma unroll
+                    for (int j = 0; j < 12; j++) z[j] = __shfl(acc[j & 3], (j >> 2) * 16 + col);
+                    const int64_t p = base + (wave * MI_C + c) * 16 + col;
+                    if (g == 0 && p < n) {
+                        double *o = init + p * 9;
+                        int j = 0;
+                        for (int k = 0; k < 9; k++) {
+                            if (k == 2) {
+                                const double sn = z[j++], cs = z[j++];
+                                double ph = 0.5 * atan2(sn, cs);
+                                if (ph < 0) ph += kPi;
+                                o[2] = ph;
+                            } else if (k == 7) {
+                                o[7] = (double)z[j++] * imax[c];
+                            } else {
+                                double u = z[j++];
+                                u = u < 0 ? 0 : (u > 1 ? 1 : u);
+                                o[k] = nm.lo[k] + u * (nm.hi[k] - nm.lo[k]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- host side
 struct MlpState {
     int nin = 0, ninp = 0, trained = 0;
@@ -279,6 +455,11 @@ struct MlpState {
     float *prof_tmp = nullptr;
     double *truth_tmp = nullptr;
     int64_t tmp_cap = 0;
+    // fused-inference weights (fragment layout), rebuilt after parameter changes
+    bf16x8 *wpk = nullptr;
+    float *bpk = nullptr;
+    MlpPack pack;
+    int packed = 0;
 };
 
 inline int64_t mlp_num_params(const MlpState &m)
@@ -292,6 +473,7 @@ inline void mlp_free(MlpState &m)
 {
     hipFree(m.P); hipFree(m.Mo); hipFree(m.Vo); hipFree(m.G); hipFree(m.Pb);
     hipFree(m.X); hipFree(m.Y); hipFree(m.loss); hipFree(m.prof_tmp); hipFree(m.truth_tmp);
+    hipFree(m.wpk); hipFree(m.bpk);
     for (int l = 0; l < MLP_NL; l++) { hipFree(m.A[l]); hipFree(m.D[l]); hipFree(m.Z[l]); }
     m = MlpState();
 }
@@ -479,6 +661,7 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
     if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) return ME_EDEVICE;
     if (final_loss) *final_loss = lossv;
     m.trained = 1;
+    m.packed = 0;
     return ME_OK;
 }
 
@@ -488,6 +671,44 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
 {
     if (!m.trained) return ME_ENOTREADY;
     const int nl = C.nl;
+    if (m.ninp <= 256) {
+        if (!m.packed) {
+            MlpPack &d = m.pack;
+            int64_t fr = 0;
+            for (int l = 0; l < MLP_NL; l++) {
+                d.offW[l] = m.offW[l];
+                d.offB[l] = m.offB[l];
+                d.din[l] = m.dims[l];
+                d.dout[l] = m.dims[l + 1];
+                d.T[l] = m.dims[l + 1] / 16;
+                d.KB[l] = m.dims[l] / 32;
+                d.frag[l] = fr;
+                fr += (int64_t)d.T[l] * d.KB[l];
+            }
+            if (!m.wpk) {
+                hipError_t e = hipMalloc(&m.wpk, sizeof(bf16x8) * 64 * fr);
+                e = e ? e : hipMalloc(&m.bpk, sizeof(float) * MLP_NL * 256);
+                if (e != hipSuccess) return ME_ENOMEM;
+            }
+            const int64_t work = fr * 64 > MLP_NL * 256 ? fr * 64 : MLP_NL * 256;
+            hipLaunchKernelGGL(mlp_pack_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s,
+                               (const __bf16 *)m.Pb, (const float *)m.P, d, fr, m.wpk, m.bpk);
+            m.packed = 1;
+        }
+        int dev = 0, ncu = 1;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const int64_t passes = (n + MI_PX - 1) / MI_PX;
+        const unsigned grid = (unsigned)(passes < ncu ? passes : ncu);
+        if (grid == 0) return ME_OK;
+        if (prof_is_f64)
+            hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(grid), dim3(256), 0, s, (const double *)prof, n, nl,
+                               m.pack.KB[0], (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
+        else
+            hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(grid), dim3(256), 0, s, (const float *)prof, n, nl,
+                               m.pack.KB[0], (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
+        return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
+    }
     const int64_t chunk = 1 << 20;
     me_status st = mlp_alloc_acts(m, n < chunk ? ((n + 63) / 64) * 64 : chunk);
     if (st != ME_OK) return st;
@@ -545,6 +766,7 @@ inline me_status mlp_set_params(MlpState &m, hipStream_t s, const float *in)
     if (hipMemcpyAsync(m.P, h.data(), sizeof(float) * m.np, hipMemcpyHostToDevice, s) != hipSuccess) return ME_EDEVICE;
     hipLaunchKernelGGL(to_bf16_kernel, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Pb, m.np);
     if (hipStreamSynchronize(s) != hipSuccess) return ME_EDEVICE;
+    m.packed = 0;
     return ME_OK;
 }
 

@@ -82,3 +82,69 @@ def test_learned_initialiser_gains_over_plain():
     assert err_b < 400 and err_v < 1.0
     assert conv_l >= conv_p + 0.1
     assert runs_l < 0.6 * runs_p
+
+
+def _numpy_predict(params, prof, nl, ranges):
+    """Forward pass of the exported parameters in FP64 with the device's bf16
+    rounding points, then the device's decoding into starting points."""
+    prof = prof.astype(np.float32)
+    imax = np.maximum(prof[:, 0, :].max(axis=1), 0).astype(np.float32)
+    inv = np.where(imax > 0, np.float32(1) / np.where(imax > 0, imax, 1), 0).astype(np.float32)
+    x = np.concatenate([prof[:, 0, :] * inv[:, None] - np.float32(1),
+                        np.float32(10) * prof[:, 1:, :].reshape(len(prof), -1) * inv[:, None]], axis=1)
+    x = np.where(np.isfinite(x), x, 0)
+    a = M.bf16(x)
+    layers = M.unpack(params.astype(np.float64), 4 * nl)
+    for l, (W, b) in enumerate(layers):
+        z = a @ M.bf16(W) + b
+        if l < 3:
+            a = M.bf16(M.gelu(z))
+    lo, hi = ranges[:, 0], np.where(ranges[:, 1] > ranges[:, 0], ranges[:, 1], ranges[:, 0] + 1)
+    out = np.empty((len(prof), 9))
+    j = 0
+    for k in range(9):
+        if k == 2:
+            ph = 0.5 * np.arctan2(z[:, j], z[:, j + 1])
+            out[:, 2] = np.where(ph < 0, ph + np.pi, ph)
+            j += 2
+        elif k == 7:
+            out[:, 7] = z[:, j] * imax
+            j += 1
+        else:
+            out[:, k] = lo[k] + np.clip(z[:, j], 0, 1) * (hi[k] - lo[k])
+            j += 1
+    return out, z
+
+
+@pytest.mark.parametrize("name", ["hmi4k", "niris64", "hinode"])
+def test_mlp_prediction_matches_numpy_forward(name):
+    """Device inference (the fused all-layer kernel for 4*nl <= 256, the
+    layer-by-layer GEMMs above) against the numpy forward pass of the same
+    exported parameters, on a pixel count that leaves a partial last tile."""
+    import torch
+    w = get(name)
+    n = 3001
+    ranges = w.ranges()
+    with gpu_inversion(name, precision=ME_FP32) as inv:
+        inv.mlp_train(w.seed + 7, ranges, 16384, batch=1024, epochs=1, lr=2e-3, ff_frac=w.ff_frac,
+                      ff_bmax=w.ff_bmax, sigma=w.sigma)
+        params = inv.mlp_params()
+        d_prof = device_tensor((n, 4, w.nl), torch.float32)
+        inv.gen_map_device(w.seed, ranges, n, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                           sigma=w.sigma)
+        d_init = device_tensor((n, 9), torch.float64)
+        d_init.fill_(-1.0)
+        inv.mlp_predict_device(d_prof.data_ptr(), n, d_init.data_ptr())
+        pred = d_init.cpu().numpy()
+        prof = d_prof.cpu().numpy()
+    ref, _ = _numpy_predict(params, prof, w.nl, ranges)
+    span = ranges[:, 1] - ranges[:, 0]
+    for k in (0, 1, 3, 4, 5, 6, 8):
+        d = np.abs(pred[:, k] - ref[:, k]) / span[k]
+        print(f"{name} param {k}: max normalised difference {d.max():.2e}, 99.9% {np.quantile(d, 0.999):.2e}")
+        assert np.quantile(d, 0.999) <= 5e-3 and d.max() <= 3e-2
+    dphi = np.abs(pred[:, 2] - ref[:, 2])
+    dphi = np.minimum(dphi, np.pi - dphi)
+    assert np.quantile(dphi, 0.999) <= 2e-2
+    ds = np.abs(pred[:, 7] - ref[:, 7]) / np.maximum(np.abs(ref[:, 7]), 1e-3)
+    assert np.quantile(ds, 0.999) <= 5e-3
