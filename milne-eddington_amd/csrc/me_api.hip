@@ -59,6 +59,28 @@ constexpr double kThr[5][5] = {{1.6e4, 160.0, 107.0, 28.5, 3.5},
                                {1.5e7, 5.01e3, 380.0, 115.0, 114.0},
                                {1.3e8, 1.6e4, 810.0, 195.0, 116.0}};
 
+// Weideman (1994) coefficients for N terms: L = (N / sqrt 2)^(1/2); with
+// t_k = L tan(k pi / 2M), f_k = exp(-t_k^2) (L^2 + t_k^2) for |k| < M = 2N
+// (f_{-M} = 0), c_n is the (n+1)-th Fourier coefficient of f over the 4N
+// points, n = 0 .. N-1.  A direct DFT of 64 points: computed once per context.
+void weideman_coefficients(int N, float &L, float *c)
+{
+    const int M = 2 * N, M2 = 2 * M;
+    const double Ld = std::sqrt(N / std::sqrt(2.0));
+    std::vector<double> g(M2, 0.0);  // g[m]: f at k = m - M (index 0 is k = -M, f = 0)
+    for (int k = -M + 1; k < M; k++) {
+        const double t = Ld * std::tan(k * M_PI / (2.0 * M));
+        g[k + M] = std::exp(-t * t) * (Ld * Ld + t * t);
+    }
+    // fftshift: h[m] = g[(m + M) % M2]; c_n = Re(sum_m h[m] e^{-2 pi i (n+1) m / M2}) / M2
+    for (int n = 0; n < N; n++) {
+        double acc = 0.0;
+        for (int m = 0; m < M2; m++) acc += g[(m + M) % M2] * std::cos(2.0 * M_PI * (n + 1) * m / M2);
+        c[n] = (float)(acc / M2);
+    }
+    L = (float)Ld;
+}
+
 me_status hip_status(hipError_t e)
 {
     if (e == hipSuccess) return ME_OK;
@@ -340,6 +362,7 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     d.fsqrtlogr = std::sqrt(d.flogr);
     int ifig = std::min(4, cfg->nfigures > 4 ? cfg->nfigures - 4 : 0);
     for (int k = 0; k < 5; k++) d.fthr[k] = kThr[ifig][k];
+    weideman_coefficients(16, d.wL, d.wc);
     d.seed = cfg->seed;
 
     double etab[50];
