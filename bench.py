@@ -28,15 +28,14 @@ sys.path.insert(0, str(REPO / "milne-eddington_amd"))
 
 # per-event algorithmic FLOP constants (one FMA = 2, a transcendental = 1)
 C_SYN, C_JAC, C_LOSS, C_HESS = 88, 1140, 16, 684
-C_F = [274, 97, 12, 20, 35, 45, 60, 4]   # FP64: 916, Hui p6, asymptotic 1-4 terms, y = 0, nu = 0
-C_W = 135                                # FP32: Weideman N = 16 rational approximation, per call
+C_F = [274, 97, 12, 20, 35, 45, 60, 4]   # Faddeeva by region: 916, Hui p6, asymptotic 1-4 terms, y = 0, nu = 0
 C_SVD, C_BK = 15309, 171
 P32, P64 = 157.3e12, 78.6e12             # MI355X dense FP32 / FP64 peaks
 
 
 def op_model(st, nl: int, fp64: bool):
     f_core = nl * (st.n_jac * (C_SYN + C_JAC + C_LOSS) + st.n_trial * (C_SYN + C_LOSS) + st.n_hess * C_HESS)
-    f_fadd = sum(n * c for n, c in zip(st.n_fadd, C_F)) if fp64 else sum(st.n_fadd) * C_W
+    f_fadd = sum(n * c for n, c in zip(st.n_fadd, C_F))
     f_svd = st.n_svd * C_SVD + st.n_bk * C_BK
     if fp64:
         return 0.0, float(f_core + f_fadd + f_svd)

@@ -99,7 +99,8 @@ typedef struct me_stats {
     int64_t n_svd;             /* 9x9 SVDs                                  */
     int64_t n_bk;              /* back-substitutions                        */
     int64_t n_fadd[8];         /* Faddeeva calls by branch: 916, Hui p6, asym1, asym2, asym3, asym4, y=0, nu=0
-                                  (ME_FP32 evaluates one rational approximation everywhere: all calls in [0]) */
+                                  (ME_FP32 evaluates one rational approximation everywhere and counts each call
+                                  by the region the FP64 evaluation would take) */
     int64_t n_svd_fail;
     int64_t n_fallback;
     double  kernel_ms;         /* inversion kernel time, HIP events on the context stream */

@@ -252,6 +252,31 @@ template <typename T> __device__ void alg916(T nu, T y, const DevCfg &C, T &H, T
     L = nu > T(0) ? im : -im;
 }
 
+// The branch slot faddeeva() below takes for (nu, y), without evaluating it:
+// the FP32 path evaluates one rational approximation everywhere but counts
+// its calls by the region the reference would have used (the op model prices
+// each region differently).
+template <typename T> __device__ __forceinline__ int faddeeva_branch(T nu, T y, const DevCfg &C)
+{
+    if (nu == T(0)) return FNU0;
+    if (y == T(0)) return FY0;
+    const T r2 = nu * nu + y * y, y2 = y * y;
+    if (r2 >= T(C.fthr[0])) return FAS1;
+    if (r2 >= T(C.fthr[1])) return FAS2;
+    if (r2 >= T(C.fthr[2])) return FAS3;
+    if (r2 >= T(C.fthr[3])) {
+        if ((C.nfig <= 4 && y2 < T(6e-14)) || (C.nfig == 5 && r2 < T(39) && y2 < T(1e-9))) return FHUI;
+        return FAS4;
+    }
+    if (r2 >= T(C.fthr[4])) {
+        if (C.nfig == 6) return y2 >= T(1) ? FHUI : F916;
+        if (C.nfig <= 4) return FHUI;
+        if (C.nfig == 5) return (y2 >= T(0.27) || y2 < T(1e-9)) ? FHUI : F916;
+        return FAS4;
+    }
+    return C.nfig <= 4 ? FHUI : F916;
+}
+
 // w(z) by region of |z|^2 and accuracy nfigures; returns the branch slot
 template <typename T> __device__ __forceinline__ int faddeeva(T nu, T y, const DevCfg &C, T &H, T &L)
 {
@@ -439,7 +464,7 @@ __device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T st
         int b;
         if constexpr (std::is_same<T, float>::value) {
             weideman16(v, K.damp, C, H, L);
-            b = F916;  // the FP32 path counts every evaluation in slot 0
+            b = fcnt ? faddeeva_branch<T>(v, K.damp, C) : 0;
         } else {
             b = faddeeva<T>(v, K.damp, C, H, L);
         }
