@@ -289,8 +289,9 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     if (cfg->block_size != 0 && (cfg->block_size < 64 || cfg->block_size > 256 || cfg->block_size % 64))
         return ME_EINVAL;  // the kernel's LDS counters hold 256 lanes
     if (cfg->precision != ME_FP64 && cfg->precision != ME_FP32) return ME_EINVAL;
-    for (int p = 0; p < 9; p++)
-        if (cfg->fixed[p]) return ME_EINVAL;  // the device path inverts all 9 parameters
+    int nfree = 0;
+    for (int p = 0; p < 9; p++) nfree += cfg->fixed[p] ? 0 : 1;
+    if (nfree == 0) return ME_EINVAL;  // nothing to invert (the reference sizes its matrices by npar)
     for (int i = 0; i < nl; i++)
         if (!std::isfinite(lambda[i])) return ME_EINVAL;
     int ndev = 0;
@@ -314,6 +315,8 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
         d.lim[p][1] = cfg->bounds[p][1];
         d.step[p] = cfg->step[p];
         d.regul[p] = cfg->regul[p];
+        d.pidx[p] = cfg->fixed[p] ? -1 : d.npar;
+        if (!cfg->fixed[p]) d.pmap[d.npar++] = p;
         d.rval[p] = cfg->regul_value[p];
         d.rw[p] = cfg->regul_weight[p];
         if (cfg->regul[p]) d.regul_any = 1;

@@ -30,6 +30,9 @@ struct DevCfg {
     int nl, ic, i0, i1, ncut;
     int niter, nruns, hmi_ref, nfig, regul_any;
     int regul[9];
+    int npar;                 // inverted (free) parameters; the rest keep their starting value
+    int pmap[9];              // packed index -> parameter index (ascending), first npar used
+    int pidx[9];              // parameter index -> packed index, -1 for a fixed parameter
     double lambda0, bshift, icrit, crit, convg, thr, damp0, lacc, lrej, llim0, llim1, vco, lco;
     double step[9];
     double lim[9][2];

@@ -164,7 +164,7 @@ __device__ __forceinline__ double penalty_of(const DevCfg &C, const double p[9])
     double pen = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        if (C.regul[k]) {
+        if (C.regul[k] && C.pidx[k] >= 0) {
             double r = p[k] - C.rval[k];
             pen += r * r * C.rw[k];
         }
@@ -297,13 +297,21 @@ struct Sol9 {
 };
 
 __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
-                                       const double *__restrict__ gv, int64_t NS, double thr0)
+                                       const double *__restrict__ gv, int64_t NS, double thr0, int npar)
 {
     Sol9 r;
     double M[81], w[9];
 #pragma unroll
     for (int k = 0; k < 81; k++) M[k] = in[(int64_t)k * NS];
     r.status = svd9(M, w);
+    // With fixed parameters the system is the packed npar x npar one, padded
+    // with zero rows and columns: every Golub-Reinsch step on the padded
+    // matrix equals the step on the packed matrix, the padding gets zero
+    // singular values (truncated) and unit columns in U.  Clearing those unit
+    // entries keeps the next trial's input (this U, H1) padded with zeros.
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        if (i >= npar) M[i * 10] = 0.0;
     double ug[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) {
@@ -358,6 +366,25 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
         thr *= 5.;
     }
     return r;
+}
+
+// Fixed parameters (Src/LM_FIT.c:316-347): keep the rows and columns of the
+// free parameters, in parameter order, packed into the top-left npar x npar
+// block of the lane's matrix slot (zeros elsewhere), and the gradient into its
+// first npar entries.  Each target entry reads a source at the same or a later
+// position, so the compaction runs in place in row-major order.
+__device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__restrict__ mat,
+                                       double *__restrict__ g, int64_t NS)
+{
+    const DevCfg &C = *Cp;
+    const int np = C.npar;
+    for (int a = 0; a < 9; a++) {
+        for (int b = 0; b < 9; b++) {
+            const double v = (a < np && b < np) ? mat[(int64_t)(C.pmap[a] * 9 + C.pmap[b]) * NS] : 0.0;
+            mat[(int64_t)(a * 9 + b) * NS] = v;
+        }
+    }
+    for (int a = 0; a < 9; a++) g[(int64_t)a * NS] = a < np ? g[(int64_t)C.pmap[a] * NS] : 0.0;
 }
 
 template <typename T, typename TIn>
@@ -474,15 +501,17 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
             }
-            // regularisation terms on the diagonal and the gradient
-            // (Src/LM_FIT.c:373-393)
+            if (C.npar < 9) pack_free(Cp, mat, scr + (int64_t)SL_G * NS, NS);
+            // regularisation terms on the diagonal and the gradient of the
+            // free parameters (Src/LM_FIT.c:373-393)
             if (C.regul_any) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) {
-                    if (C.regul[k]) {
+                    const int ip = C.pidx[k];
+                    if (C.regul[k] && ip >= 0) {
                         const double r = par[k] - C.rval[k];
-                        SCR(SL_G + k) -= (r * C.rw[k]) * ratio;
-                        mat[(int64_t)(k * 10) * NS] += C.rw[k] * ratio;
+                        scr[(int64_t)(SL_G + ip) * NS] -= (r * C.rw[k]) * ratio;
+                        mat[(int64_t)(ip * 10) * NS] += C.rw[k] * ratio;
                     }
                 }
             }
@@ -497,7 +526,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             // rejected trial: H1); U goes to the other slot and becomes the
             // next trial's input.  One call site for every lane of the wave.
             const Sol9 sv = svd_solve(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
-                                      scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr);
+                                      scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr, C.npar);
             CT(CT_SVD) += 1;
             CT(CT_BK) += sv.nbk;
             if (sv.status) { px_status |= 2; CT(CT_SVDFAIL) += 1; }
@@ -507,7 +536,22 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             mbuf ^= 1;
             double pt[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) pt[k] = par[k] + sol[k] * (k == 7 ? step7 : C.step[k]);
+            for (int k = 0; k < 9; k++) pt[k] = par[k];
+            if (C.npar == 9) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) pt[k] += sol[k] * (k == 7 ? step7 : C.step[k]);
+            } else {
+                // packed step (Par_update, Src/LM_FIT.c:226-233)
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    const int ip = C.pidx[k];
+                    double sk = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 9; a++)
+                        if (a == ip) sk = sol[a];
+                    if (ip >= 0) pt[k] += sk * (k == 7 ? step7 : C.step[k]);
+                }
+            }
             bounds_check(C, pt, l7lo, l7hi);
 
             P9 pv;

@@ -225,22 +225,32 @@ class Oracle:
                         cnt.ctypes.data_as(C.POINTER(C.c_int64)))
         return syn, cnt
 
+    @property
+    def npar(self):
+        """Inverted parameters: the normal equations are npar x npar, packed
+        in parameter order (Src/LM_FIT.c:316-347)."""
+        return sum(0 if self.cfg.fixed[k] else 1 for k in range(9))
+
     def normal_eq(self, par, prof, step7, norm0):
         par = np.ascontiguousarray(par, dtype=np.float64)
         prof = np.ascontiguousarray(prof, dtype=np.float64).reshape(4, self.nl)
-        H = np.empty((9, 9))
+        n = self.npar
+        H = np.empty(81)
         g = np.empty(9)
         chi = C.c_double()
         lib().orc_normal_eq(self.h, _ptr(par, _dp), _ptr(prof, _dp), step7, norm0, _ptr(H, _dp),
                             _ptr(g, _dp), C.byref(chi))
-        return H, g, chi.value
+        return H[:n * n].reshape(n, n).copy(), g[:n].copy(), chi.value
 
     def solve(self, H, g):
-        H = np.array(H, dtype=np.float64, order="C")
-        g = np.ascontiguousarray(g, dtype=np.float64)
+        n = self.npar
+        Hb = np.zeros(81)
+        Hb[:n * n] = np.asarray(H, dtype=np.float64).reshape(n * n)
+        gb = np.zeros(9)
+        gb[:n] = np.asarray(g, dtype=np.float64).reshape(n)
         sol = np.empty(9)
-        lib().orc_solve(self.h, _ptr(H, _dp), _ptr(g, _dp), _ptr(sol, _dp))
-        return sol, H
+        lib().orc_solve(self.h, _ptr(Hb, _dp), _ptr(gb, _dp), _ptr(sol, _dp))
+        return sol[:n].copy(), Hb[:n * n].reshape(n, n).copy()
 
     def init_guess(self, prof_batch, pix=0):
         prof_batch = np.ascontiguousarray(prof_batch, dtype=np.float64).reshape(-1, 4, self.nl)

@@ -154,3 +154,19 @@ def test_fit_profiles_are_the_best_model():
         out = inv.invert(prof, fit=True)
         syn = inv.synth(out["res"][:, :9])
     np.testing.assert_allclose(out["fit"], syn, atol=1e-14)
+
+
+@pytest.mark.parametrize("fixed", [(5, 8), (1, 2), (0, 4, 6, 7)])
+@pytest.mark.parametrize("name", ["niris64", "hmi4k"])
+def test_inversion_fp64_parity_fixed_parameters(name, fixed):
+    """Fixed parameters (npar < 9): the device packs the free rows/columns
+    into a zero-padded 9x9 system; agreement with the oracle's packed
+    npar x npar path at the two-trial cap and under the default policy."""
+    fx = tuple(1 if k in fixed else 0 for k in range(9))
+    n = 512
+    ok2, floor2, out2, _ = _parity(name, n, "heuristic", fixed=fx, lambda_limits=(1e-5, 6e-3))
+    ok, floor, out, ref = _parity(name, n, "heuristic", fixed=fx)
+    print(f"{name} fixed {fixed}: two-trial cap {ok2.mean():.4f} (oracle self {floor2:.4f}); default "
+          f"{ok.mean():.4f} (oracle self {floor:.4f})")
+    assert ok2.mean() >= floor2 - 0.02
+    assert ok.mean() >= floor - 0.03
