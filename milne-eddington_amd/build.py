@@ -20,7 +20,35 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wno-unused-result"]
 
 
+HOST = HERE / "host"
+HOST_BIN = HOST / "meinv"
+
+
+def build_host(verbose: bool = False) -> Path:
+    """The MEINV-style host program (plain C++ over the C ABI), linked
+    against the in-tree libmeinv.so."""
+    cxx = os.environ.get("CXX", "g++")
+    srcs = [str(HOST / f) for f in ("meinv_main.cpp", "fits.cpp", "control.cpp")]
+    tmp = HOST_BIN.with_suffix(".tmp")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", str(tmp), *srcs, f"-L{CSRC}", "-lmeinv",
+           "-Wl,-rpath,$ORIGIN/../csrc", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the meinv host program failed")
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
+
+
 def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
+    lib = build_lib(verbose, extra)
+    build_host(verbose)
+    return lib
+
+
+def build_lib(verbose: bool = False, extra: list[str] | None = None) -> Path:
     srcs = [str(CSRC / "me_api.hip")]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, *FLAGS, *(extra or []), "-o", str(tmp), *srcs]

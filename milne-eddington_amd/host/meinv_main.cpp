@@ -1,0 +1,345 @@
+// meinv — host program in place of MEINV's main (Src/ME_INV.c:64-297):
+// control file -> wavelength and Stokes cube (FITS) -> inversion of the
+// solution box on the GPU through libmeinv -> res.fits [10, nx, ny] and
+// optionally fit.fits [nl, 4, nx, ny], with MEINV's chunking (nprofiles),
+// Cache_prof and the resumable progress file of Cache_inv (Src/IO.c:480-740).
+//
+//   meinv [control-file] [--precision fp64|fp32] [--batch N] [--max-chunks K]
+//   meinv --dump-config [control-file]      parsed settings as JSON, no GPU
+//   meinv --fits-info FILE                  header summary as JSON, no GPU
+//   meinv --fits-read FILE FIRST COUNT      pixels as text, no GPU
+//   meinv --fits-demo FILE                  writes a 3x4x2 BITPIX -64 image holding
+//                                           0..23 in two partial writes, no GPU
+//
+// The control file defaults to ./input, as in MEINV.  Chunks of nprofiles
+// pixels are the unit of the progress file; consecutive chunks are inverted
+// together in GPU batches of up to --batch pixels (default 1048576).
+// --max-chunks K stops after K chunks as if the run had been interrupted.
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/meinv.h"
+#include "control.hpp"
+#include "fits.hpp"
+
+namespace {
+
+int fail(const std::string &msg)
+{
+    std::fprintf(stderr, "meinv: %s\n", msg.c_str());
+    return 1;
+}
+
+double now_s()
+{
+    timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+void json_array(std::string &s, const double *v, int n)
+{
+    s += "[";
+    char b[64];
+    for (int i = 0; i < n; i++) {
+        std::snprintf(b, sizeof(b), "%s%.17g", i ? ", " : "", v[i]);
+        s += b;
+    }
+    s += "]";
+}
+
+int dump_config(const std::string &path)
+{
+    mehost::Control ctl;
+    std::string err;
+    if (!mehost::parse_control(path, ctl, err)) return fail(err);
+    const me_config &c = ctl.cfg;
+    std::string s = "{";
+    char b[512];
+    std::snprintf(b, sizeof(b),
+                  "\"lambda0\": %.17g, \"geff\": %.17g, \"num_iter\": %d, \"num_run\": %d, \"convg_criteria\": %.17g, "
+                  "\"chisq_criteria\": %.17g, \"damp\": %.17g, \"lambda_accept\": %.17g, \"lambda_reject\": %.17g, ",
+                  c.lambda0, c.geff, c.num_iter, c.num_run, c.convg_criteria, c.chisq_criteria, c.damp,
+                  c.lambda_accept, c.lambda_reject);
+    s += b;
+    s += "\"weights\": ";
+    json_array(s, c.weights, 4);
+    s += ", \"lambda_limits\": ";
+    json_array(s, c.lambda_limits, 2);
+    double bnd[18], fx[9], rg[9];
+    for (int p = 0; p < 9; p++) {
+        bnd[2 * p] = c.bounds[p][0];
+        bnd[2 * p + 1] = c.bounds[p][1];
+        fx[p] = c.fixed[p];
+        rg[p] = c.regul[p];
+    }
+    s += ", \"bounds\": ";
+    json_array(s, bnd, 18);
+    s += ", \"fixed\": ";
+    json_array(s, fx, 9);
+    s += ", \"regul\": ";
+    json_array(s, rg, 9);
+    s += ", \"regul_value\": ";
+    json_array(s, c.regul_value, 9);
+    s += ", \"regul_weight\": ";
+    json_array(s, c.regul_weight, 9);
+    s += ", \"step\": ";
+    json_array(s, c.step, 9);
+    std::snprintf(b, sizeof(b),
+                  ", \"vcoeffi\": %.17g, \"lcoeffi\": %.17g, \"icriteria\": %.17g, \"svd_threshold\": %.17g, "
+                  "\"hmi_ref\": %d, \"nfigures\": %d, \"output_fit\": %d, \"sol_box\": [%d, %d, %d, %d], "
+                  "\"nprofiles\": %d, \"cache_prof\": %d, \"cache_inv\": %d, ",
+                  c.vcoeffi, c.lcoeffi, c.icriteria, c.svd_threshold, c.hmi_ref, c.nfigures, c.output_fit,
+                  ctl.sol_box[0][0], ctl.sol_box[0][1], ctl.sol_box[1][0], ctl.sol_box[1][1], ctl.nprofiles,
+                  ctl.cache_prof ? 1 : 0, ctl.cache_inv ? 1 : 0);
+    s += b;
+    s += "\"data_path\": \"" + ctl.data_path + "\", \"wavelength_path\": \"" + ctl.wavelength_path +
+         "\", \"result_path\": \"" + ctl.result_path + "\", \"fit_path\": \"" + ctl.fit_path +
+         "\", \"cache_path\": \"" + ctl.cache_path + "\", \"warnings\": " + std::to_string(ctl.warnings.size()) +
+         "}";
+    std::printf("%s\n", s.c_str());
+    return 0;
+}
+
+int fits_info(const std::string &path)
+{
+    mefits::Image img;
+    std::string err;
+    if (!mefits::open_image(path, img, err)) return fail(err);
+    std::printf("{\"bitpix\": %d, \"naxes\": [", img.bitpix);
+    for (size_t k = 0; k < img.naxes.size(); k++) std::printf("%s%" PRId64, k ? ", " : "", img.naxes[k]);
+    std::printf("], \"bscale\": %.17g, \"bzero\": %.17g, \"data_offset\": %" PRId64 "}\n", img.bscale, img.bzero,
+                img.data_offset);
+    return 0;
+}
+
+int fits_read(const std::string &path, int64_t first, int64_t count)
+{
+    mefits::Image img;
+    std::string err;
+    if (!mefits::open_image(path, img, err)) return fail(err);
+    std::vector<double> v((size_t)count);
+    if (!mefits::read_pixels(img, first, count, v.data(), err)) return fail(err);
+    for (double x : v) std::printf("%.17g\n", x);
+    return 0;
+}
+
+int fits_demo(const std::string &path)
+{
+    mefits::Image img;
+    std::string err;
+    if (!mefits::create_image(path, {3, 4, 2}, img, err)) return fail(err);
+    std::vector<double> v(24);
+    for (int i = 0; i < 24; i++) v[(size_t)i] = i;
+    if (!mefits::write_pixels(img, 10, 14, v.data() + 10, err)) return fail(err);
+    if (!mefits::write_pixels(img, 0, 10, v.data(), err)) return fail(err);
+    return 0;
+}
+
+// progress file: the reference's packed header {"cach", nx, ny, ncache}
+// followed by one int flag per chunk (Src/RINPUT.h:97-104, Src/IO.c:480-540)
+struct CacheHeader {
+    char magic[4];
+    int32_t nx, ny, ncache;
+} __attribute__((packed));
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    std::string control = "./input";
+    int prec = ME_FP64;
+    int64_t batch = 1 << 20;
+    int64_t max_chunks = -1;
+    bool have_control = false;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "--dump-config") return dump_config(i + 1 < argc ? argv[i + 1] : "./input");
+        if (a == "--fits-info" && i + 1 < argc) return fits_info(argv[i + 1]);
+        if (a == "--fits-demo" && i + 1 < argc) return fits_demo(argv[i + 1]);
+        if (a == "--fits-read" && i + 3 < argc) return fits_read(argv[i + 1], std::atoll(argv[i + 2]), std::atoll(argv[i + 3]));
+        if (a == "--precision" && i + 1 < argc) {
+            std::string p = argv[++i];
+            if (p == "fp32") prec = ME_FP32;
+            else if (p == "fp64") prec = ME_FP64;
+            else return fail("--precision fp64|fp32");
+        } else if (a == "--batch" && i + 1 < argc) {
+            batch = std::atoll(argv[++i]);
+            if (batch < 1) return fail("--batch N needs N >= 1");
+        } else if (a == "--max-chunks" && i + 1 < argc) {
+            max_chunks = std::atoll(argv[++i]);
+        } else if (!a.empty() && a[0] != '-' && !have_control) {
+            control = a;
+            have_control = true;
+        } else {
+            return fail("unknown argument " + a);
+        }
+    }
+
+    mehost::Control ctl;
+    std::string err;
+    if (!mehost::parse_control(control, ctl, err)) return fail(err);
+    for (const std::string &w : ctl.warnings)
+        if (ctl.verbose > 0) std::fprintf(stderr, "meinv: %s\n", w.c_str());
+
+    // wavelengths and cube geometry (Src/IO.c:120-286)
+    mefits::Image cube, wav;
+    if (!mefits::open_image(ctl.data_path, cube, err)) return fail(err);
+    if (!mefits::open_image(ctl.wavelength_path, wav, err)) return fail(err);
+    const int naxis = (int)cube.naxes.size();
+    if (naxis < 3 || naxis > 4) return fail("wrong dimension of the data hdu.");
+    if (cube.naxes[1] != 4) return fail("wrong size of the data hdu.");
+    if (cube.bitpix != 16 && cube.bitpix != -32 && cube.bitpix != -64)
+        return fail("error in the data type of data hdu: bitpix = " + std::to_string(cube.bitpix));
+    const int nl = (int)cube.naxes[0];
+    const int NX = (int)cube.naxes[2], NY = naxis == 3 ? 1 : (int)cube.naxes[3];
+    if (wav.naxes.size() != 1) return fail("error in the dimension of wavelength hdu.");
+    if (wav.naxes[0] != nl) return fail("wavelength is not consistent with the profile.");
+    if (wav.bitpix != -32 && wav.bitpix != -64) return fail("error in the data type of the wavelength hdu.");
+    std::vector<double> lambda((size_t)nl);
+    if (!mefits::read_pixels(wav, 0, nl, lambda.data(), err)) return fail(err);
+
+    int (&box)[2][2] = ctl.sol_box;
+    if (box[0][0] < 0 || box[0][0] >= NX) box[0][0] = 0;
+    if (box[0][1] < 0 || box[0][1] >= NX) box[0][1] = NX - 1;
+    if (box[1][0] < 0 || box[1][0] >= NY) box[1][0] = 0;
+    if (box[1][1] < 0 || box[1][1] >= NY) box[1][1] = NY - 1;
+    const int nxb = box[0][1] - box[0][0] + 1, nyb = box[1][1] - box[1][0] + 1;
+    if (nxb < 1 || nyb < 1) return fail("empty solution box");
+    const int64_t counts = (int64_t)nxb * nyb;
+    int64_t nprof = ctl.nprofiles;
+    if (nprof < 1) nprof = 1;
+    if (nprof > counts) nprof = counts;
+    if (ctl.cache_prof) nprof = counts;
+    const int64_t ncache = counts / nprof + 1;  // as INIT_INV sizes it (Src/INV_INIT.c:230-231)
+    const bool output_fit = ctl.cfg.output_fit != 0;
+
+    // progress file and result files (Src/IO.c:480-669)
+    std::vector<int32_t> flags((size_t)ncache, 0);
+    const bool use_cache = ctl.cache_inv && !ctl.cache_prof;
+    bool resume = false;
+    if (use_cache) {
+        if (FILE *f = std::fopen(ctl.cache_path.c_str(), "rb")) {
+            CacheHeader h;
+            if (std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, "cach", 4) == 0 && h.nx == nxb &&
+                h.ny == nyb && h.ncache == ncache &&
+                std::fread(flags.data(), sizeof(int32_t), (size_t)ncache, f) == (size_t)ncache)
+                resume = true;
+            else
+                std::fill(flags.begin(), flags.end(), 0);
+            std::fclose(f);
+        }
+    }
+    mefits::Image res_img, fit_img;
+    if (resume) {
+        if (!mefits::open_image(ctl.result_path, res_img, err)) return fail(err);
+        if (res_img.naxes.size() != 3 || res_img.naxes[0] != 10 || res_img.naxes[1] != nxb || res_img.naxes[2] != nyb)
+            return fail("wrong size of the result file");
+        if (output_fit) {
+            if (!mefits::open_image(ctl.fit_path, fit_img, err)) return fail(err);
+            if (fit_img.naxes.size() != 4 || fit_img.naxes[0] != nl || fit_img.naxes[1] != 4 ||
+                fit_img.naxes[2] != nxb || fit_img.naxes[3] != nyb)
+                return fail("wrong size of the fit file.");
+        }
+    } else {
+        if (use_cache) {
+            FILE *f = std::fopen(ctl.cache_path.c_str(), "wb");
+            if (!f) return fail("cannot create the cache file " + ctl.cache_path);
+            CacheHeader h;
+            std::memcpy(h.magic, "cach", 4);
+            h.nx = nxb;
+            h.ny = nyb;
+            h.ncache = (int32_t)ncache;
+            bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+                      std::fwrite(flags.data(), sizeof(int32_t), (size_t)ncache, f) == (size_t)ncache;
+            std::fclose(f);
+            if (!ok) return fail("cannot write the cache file");
+        }
+        if (!mefits::create_image(ctl.result_path, {10, nxb, nyb}, res_img, err)) return fail(err);
+        if (output_fit && !mefits::create_image(ctl.fit_path, {nl, 4, nxb, nyb}, fit_img, err)) return fail(err);
+    }
+
+    // engine
+    me_config cfg = ctl.cfg;
+    cfg.precision = prec;
+    me_ctx *ctx = nullptr;
+    me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
+    if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");
+
+    const int64_t bmax = std::max<int64_t>(batch, nprof);
+    std::vector<double> prof, res, fit;
+    int64_t done_chunks = 0, inverted = 0, converged = 0, valid = 0;
+    double t0 = now_s(), kernel_ms = 0;
+    int rc = 0;
+    for (int64_t c0 = 0; c0 < ncache && rc == 0;) {
+        if (flags[(size_t)c0] || c0 * nprof >= counts) {
+            c0++;
+            continue;
+        }
+        if (max_chunks >= 0 && done_chunks >= max_chunks) break;
+        // chunk c0 and the consecutive chunks still to do after it, up to
+        // bmax pixels (and up to --max-chunks chunks in all)
+        int64_t c1 = c0 + 1;
+        while (c1 < ncache && !flags[(size_t)c1] && c1 * nprof < counts && (c1 + 1 - c0) * nprof <= bmax &&
+               (max_chunks < 0 || done_chunks + (c1 - c0) < max_chunks))
+            c1++;
+        const int64_t p0 = c0 * nprof, p1 = std::min(counts, c1 * nprof), n = p1 - p0;
+        prof.resize((size_t)(n * 4 * nl));
+        res.resize((size_t)(n * 10));
+        if (output_fit) fit.resize((size_t)(n * 4 * nl));
+        // read the box pixels [p0, p1) in scan order, one run per image row
+        for (int64_t p = p0; p < p1 && rc == 0;) {
+            const int64_t bx = p % nxb, by = p / nxb;
+            const int64_t run = std::min<int64_t>(nxb - bx, p1 - p);
+            const int64_t fpix = ((int64_t)(box[1][0] + by) * NX + box[0][0] + bx) * 4 * nl;
+            if (!mefits::read_pixels(cube, fpix, run * 4 * nl, prof.data() + (p - p0) * 4 * nl, err)) rc = fail(err);
+            p += run;
+        }
+        if (rc) break;
+        me_stats stats;
+        std::memset(&stats, 0, sizeof(stats));
+        st = me_invert_batch_f64(ctx, prof.data(), n, res.data(), output_fit ? fit.data() : nullptr, nullptr, &stats);
+        if (st != ME_OK) {
+            rc = fail("me_invert_batch_f64 failed (status " + std::to_string((int)st) + ")");
+            break;
+        }
+        kernel_ms += stats.kernel_ms;
+        if (!mefits::write_pixels(res_img, p0 * 10, n * 10, res.data(), err)) { rc = fail(err); break; }
+        if (output_fit && !mefits::write_pixels(fit_img, p0 * 4 * nl, n * 4 * nl, fit.data(), err)) {
+            rc = fail(err);
+            break;
+        }
+        if (use_cache) {
+            FILE *f = std::fopen(ctl.cache_path.c_str(), "r+b");
+            if (!f) { rc = fail("cannot update the cache file"); break; }
+            const int32_t one = 1;
+            for (int64_t c = c0; c < c1; c++) {
+                flags[(size_t)c] = 1;
+                std::fseek(f, (long)(sizeof(CacheHeader) + c * sizeof(int32_t)), SEEK_SET);
+                std::fwrite(&one, sizeof(one), 1, f);
+            }
+            std::fclose(f);
+        }
+        done_chunks += c1 - c0;
+        inverted += n;
+        valid += stats.n_valid;
+        converged += stats.n_converged;
+        c0 = c1;
+    }
+    me_ctx_destroy(ctx);
+    if (rc) return rc;
+    const double dt = now_s() - t0;
+    std::printf("meinv: box %dx%d (%" PRId64 " px), inverted %" PRId64 " px in %" PRId64 " chunks, valid %" PRId64
+                ", converged %" PRId64 " (chi2 <= %g); %.2f s wall, %.1f ms kernel%s\n",
+                nxb, nyb, counts, inverted, done_chunks, valid, converged, ctl.cfg.chisq_criteria, dt, kernel_ms,
+                resume ? " (resumed)" : "");
+    return 0;
+}

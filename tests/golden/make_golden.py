@@ -11,11 +11,17 @@ reference's outputs):
                      chain U_k = left factor of svd(U_{k-1}) that trial k of an
                      LM iteration solves with
   niris_wavelength.npy  the 34-sample NIRIS grid (FITS BITPIX -64, big-endian)
+  niriswavelength.fits  that file itself (a real CFITSIO-written FITS image,
+                     read by the host program's FITS reader in the tests)
+  control/*_input    the reference's example control files (data: keyword =
+                     value text), parsed by the host program's control-file
+                     parser and by meinv.control in the tests
 
 Usage:  python tests/golden/make_golden.py
 """
 from __future__ import annotations
 
+import shutil
 import sys
 from pathlib import Path
 
@@ -140,6 +146,15 @@ def main():
 
     if REF_FITS.exists():
         np.save(HERE / "niris_wavelength.npy", read_fits_1d(REF_FITS))
+        shutil.copyfile(REF_FITS, HERE / "niriswavelength.fits")
+    (HERE / "control").mkdir(exist_ok=True)
+    ref = REF_FITS.parents[1]
+    for src, dst in ((ref / "niris_example" / "input", "niris_example_input"),
+                     (ref / "examples" / "niris_example" / "input", "niris_input"),
+                     (ref / "examples" / "hinode_example" / "input", "hinode_input"),
+                     (ref / "examples" / "hmi_example" / "input", "hmi_input")):
+        if src.exists():
+            shutil.copyfile(src, HERE / "control" / dst)
     print("wrote", sorted(p.name for p in HERE.iterdir() if p.suffix in (".npz", ".npy")))
 
 
