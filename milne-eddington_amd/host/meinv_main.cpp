@@ -5,6 +5,7 @@
 // Cache_prof and the resumable progress file of Cache_inv (Src/IO.c:480-740).
 //
 //   meinv [control-file] [--precision fp64|fp32] [--batch N] [--max-chunks K]
+//         [--files CUBE.fits ... [--result-dir DIR]]
 //   meinv --dump-config [control-file]      parsed settings as JSON, no GPU
 //   meinv --fits-info FILE                  header summary as JSON, no GPU
 //   meinv --fits-read FILE FIRST COUNT      pixels as text, no GPU
@@ -15,6 +16,9 @@
 // pixels are the unit of the progress file; consecutive chunks are inverted
 // together in GPU batches of up to --batch pixels (default 1048576).
 // --max-chunks K stops after K chunks as if the run had been interrupted.
+// --files inverts several cubes on the control file's wavelength grid with one
+// context (DIR/<stem>_res.fits, _fit.fits, _cache), the loop that
+// Tools/RunMEINV.py runs as one MEINV process per file.
 #include <sys/time.h>
 
 #include <algorithm>
@@ -151,61 +155,33 @@ struct CacheHeader {
     int32_t nx, ny, ncache;
 } __attribute__((packed));
 
-}  // namespace
-
-int main(int argc, char **argv)
-{
-    std::string control = "./input";
+struct Options {
     int prec = ME_FP64;
     int64_t batch = 1 << 20;
     int64_t max_chunks = -1;
-    bool have_control = false;
-    for (int i = 1; i < argc; i++) {
-        std::string a = argv[i];
-        if (a == "--dump-config") return dump_config(i + 1 < argc ? argv[i + 1] : "./input");
-        if (a == "--fits-info" && i + 1 < argc) return fits_info(argv[i + 1]);
-        if (a == "--fits-demo" && i + 1 < argc) return fits_demo(argv[i + 1]);
-        if (a == "--fits-read" && i + 3 < argc) return fits_read(argv[i + 1], std::atoll(argv[i + 2]), std::atoll(argv[i + 3]));
-        if (a == "--precision" && i + 1 < argc) {
-            std::string p = argv[++i];
-            if (p == "fp32") prec = ME_FP32;
-            else if (p == "fp64") prec = ME_FP64;
-            else return fail("--precision fp64|fp32");
-        } else if (a == "--batch" && i + 1 < argc) {
-            batch = std::atoll(argv[++i]);
-            if (batch < 1) return fail("--batch N needs N >= 1");
-        } else if (a == "--max-chunks" && i + 1 < argc) {
-            max_chunks = std::atoll(argv[++i]);
-        } else if (!a.empty() && a[0] != '-' && !have_control) {
-            control = a;
-            have_control = true;
-        } else {
-            return fail("unknown argument " + a);
-        }
-    }
+};
 
-    mehost::Control ctl;
+struct Paths {
+    std::string data, result, fit, cache;
+};
+
+// one data cube -> its result files (Src/ME_INV.c:100-135 for one process,
+// Src/IO.c:120-740); the context is shared by every cube of a run
+int invert_cube(const mehost::Control &ctl_in, const Options &opt, const Paths &pa, const std::vector<double> &lambda,
+                me_ctx *ctx)
+{
+    mehost::Control ctl = ctl_in;
     std::string err;
-    if (!mehost::parse_control(control, ctl, err)) return fail(err);
-    for (const std::string &w : ctl.warnings)
-        if (ctl.verbose > 0) std::fprintf(stderr, "meinv: %s\n", w.c_str());
-
-    // wavelengths and cube geometry (Src/IO.c:120-286)
-    mefits::Image cube, wav;
-    if (!mefits::open_image(ctl.data_path, cube, err)) return fail(err);
-    if (!mefits::open_image(ctl.wavelength_path, wav, err)) return fail(err);
+    mefits::Image cube;
+    if (!mefits::open_image(pa.data, cube, err)) return fail(err);
     const int naxis = (int)cube.naxes.size();
     if (naxis < 3 || naxis > 4) return fail("wrong dimension of the data hdu.");
     if (cube.naxes[1] != 4) return fail("wrong size of the data hdu.");
     if (cube.bitpix != 16 && cube.bitpix != -32 && cube.bitpix != -64)
         return fail("error in the data type of data hdu: bitpix = " + std::to_string(cube.bitpix));
     const int nl = (int)cube.naxes[0];
+    if (nl != (int)lambda.size()) return fail("wavelength is not consistent with the profile.");
     const int NX = (int)cube.naxes[2], NY = naxis == 3 ? 1 : (int)cube.naxes[3];
-    if (wav.naxes.size() != 1) return fail("error in the dimension of wavelength hdu.");
-    if (wav.naxes[0] != nl) return fail("wavelength is not consistent with the profile.");
-    if (wav.bitpix != -32 && wav.bitpix != -64) return fail("error in the data type of the wavelength hdu.");
-    std::vector<double> lambda((size_t)nl);
-    if (!mefits::read_pixels(wav, 0, nl, lambda.data(), err)) return fail(err);
 
     int (&box)[2][2] = ctl.sol_box;
     if (box[0][0] < 0 || box[0][0] >= NX) box[0][0] = 0;
@@ -227,7 +203,7 @@ int main(int argc, char **argv)
     const bool use_cache = ctl.cache_inv && !ctl.cache_prof;
     bool resume = false;
     if (use_cache) {
-        if (FILE *f = std::fopen(ctl.cache_path.c_str(), "rb")) {
+        if (FILE *f = std::fopen(pa.cache.c_str(), "rb")) {
             CacheHeader h;
             if (std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, "cach", 4) == 0 && h.nx == nxb &&
                 h.ny == nyb && h.ncache == ncache &&
@@ -240,19 +216,19 @@ int main(int argc, char **argv)
     }
     mefits::Image res_img, fit_img;
     if (resume) {
-        if (!mefits::open_image(ctl.result_path, res_img, err)) return fail(err);
+        if (!mefits::open_image(pa.result, res_img, err)) return fail(err);
         if (res_img.naxes.size() != 3 || res_img.naxes[0] != 10 || res_img.naxes[1] != nxb || res_img.naxes[2] != nyb)
             return fail("wrong size of the result file");
         if (output_fit) {
-            if (!mefits::open_image(ctl.fit_path, fit_img, err)) return fail(err);
+            if (!mefits::open_image(pa.fit, fit_img, err)) return fail(err);
             if (fit_img.naxes.size() != 4 || fit_img.naxes[0] != nl || fit_img.naxes[1] != 4 ||
                 fit_img.naxes[2] != nxb || fit_img.naxes[3] != nyb)
                 return fail("wrong size of the fit file.");
         }
     } else {
         if (use_cache) {
-            FILE *f = std::fopen(ctl.cache_path.c_str(), "wb");
-            if (!f) return fail("cannot create the cache file " + ctl.cache_path);
+            FILE *f = std::fopen(pa.cache.c_str(), "wb");
+            if (!f) return fail("cannot create the cache file " + pa.cache);
             CacheHeader h;
             std::memcpy(h.magic, "cach", 4);
             h.nx = nxb;
@@ -263,63 +239,49 @@ int main(int argc, char **argv)
             std::fclose(f);
             if (!ok) return fail("cannot write the cache file");
         }
-        if (!mefits::create_image(ctl.result_path, {10, nxb, nyb}, res_img, err)) return fail(err);
-        if (output_fit && !mefits::create_image(ctl.fit_path, {nl, 4, nxb, nyb}, fit_img, err)) return fail(err);
+        if (!mefits::create_image(pa.result, {10, nxb, nyb}, res_img, err)) return fail(err);
+        if (output_fit && !mefits::create_image(pa.fit, {nl, 4, nxb, nyb}, fit_img, err)) return fail(err);
     }
 
-    // engine
-    me_config cfg = ctl.cfg;
-    cfg.precision = prec;
-    me_ctx *ctx = nullptr;
-    me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
-    if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");
-
-    const int64_t bmax = std::max<int64_t>(batch, nprof);
+    const int64_t bmax = std::max<int64_t>(opt.batch, nprof);
     std::vector<double> prof, res, fit;
     int64_t done_chunks = 0, inverted = 0, converged = 0, valid = 0;
     double t0 = now_s(), kernel_ms = 0;
-    int rc = 0;
-    for (int64_t c0 = 0; c0 < ncache && rc == 0;) {
+    for (int64_t c0 = 0; c0 < ncache;) {
         if (flags[(size_t)c0] || c0 * nprof >= counts) {
             c0++;
             continue;
         }
-        if (max_chunks >= 0 && done_chunks >= max_chunks) break;
+        if (opt.max_chunks >= 0 && done_chunks >= opt.max_chunks) break;
         // chunk c0 and the consecutive chunks still to do after it, up to
         // bmax pixels (and up to --max-chunks chunks in all)
         int64_t c1 = c0 + 1;
         while (c1 < ncache && !flags[(size_t)c1] && c1 * nprof < counts && (c1 + 1 - c0) * nprof <= bmax &&
-               (max_chunks < 0 || done_chunks + (c1 - c0) < max_chunks))
+               (opt.max_chunks < 0 || done_chunks + (c1 - c0) < opt.max_chunks))
             c1++;
         const int64_t p0 = c0 * nprof, p1 = std::min(counts, c1 * nprof), n = p1 - p0;
         prof.resize((size_t)(n * 4 * nl));
         res.resize((size_t)(n * 10));
         if (output_fit) fit.resize((size_t)(n * 4 * nl));
-        // read the box pixels [p0, p1) in scan order, one run per image row
-        for (int64_t p = p0; p < p1 && rc == 0;) {
+        // the box pixels [p0, p1) in scan order, one read per image row
+        for (int64_t p = p0; p < p1;) {
             const int64_t bx = p % nxb, by = p / nxb;
             const int64_t run = std::min<int64_t>(nxb - bx, p1 - p);
             const int64_t fpix = ((int64_t)(box[1][0] + by) * NX + box[0][0] + bx) * 4 * nl;
-            if (!mefits::read_pixels(cube, fpix, run * 4 * nl, prof.data() + (p - p0) * 4 * nl, err)) rc = fail(err);
+            if (!mefits::read_pixels(cube, fpix, run * 4 * nl, prof.data() + (p - p0) * 4 * nl, err)) return fail(err);
             p += run;
         }
-        if (rc) break;
         me_stats stats;
         std::memset(&stats, 0, sizeof(stats));
-        st = me_invert_batch_f64(ctx, prof.data(), n, res.data(), output_fit ? fit.data() : nullptr, nullptr, &stats);
-        if (st != ME_OK) {
-            rc = fail("me_invert_batch_f64 failed (status " + std::to_string((int)st) + ")");
-            break;
-        }
+        me_status st =
+            me_invert_batch_f64(ctx, prof.data(), n, res.data(), output_fit ? fit.data() : nullptr, nullptr, &stats);
+        if (st != ME_OK) return fail("me_invert_batch_f64 failed (status " + std::to_string((int)st) + ")");
         kernel_ms += stats.kernel_ms;
-        if (!mefits::write_pixels(res_img, p0 * 10, n * 10, res.data(), err)) { rc = fail(err); break; }
-        if (output_fit && !mefits::write_pixels(fit_img, p0 * 4 * nl, n * 4 * nl, fit.data(), err)) {
-            rc = fail(err);
-            break;
-        }
+        if (!mefits::write_pixels(res_img, p0 * 10, n * 10, res.data(), err)) return fail(err);
+        if (output_fit && !mefits::write_pixels(fit_img, p0 * 4 * nl, n * 4 * nl, fit.data(), err)) return fail(err);
         if (use_cache) {
-            FILE *f = std::fopen(ctl.cache_path.c_str(), "r+b");
-            if (!f) { rc = fail("cannot update the cache file"); break; }
+            FILE *f = std::fopen(pa.cache.c_str(), "r+b");
+            if (!f) return fail("cannot update the cache file");
             const int32_t one = 1;
             for (int64_t c = c0; c < c1; c++) {
                 flags[(size_t)c] = 1;
@@ -334,12 +296,94 @@ int main(int argc, char **argv)
         converged += stats.n_converged;
         c0 = c1;
     }
-    me_ctx_destroy(ctx);
-    if (rc) return rc;
     const double dt = now_s() - t0;
-    std::printf("meinv: box %dx%d (%" PRId64 " px), inverted %" PRId64 " px in %" PRId64 " chunks, valid %" PRId64
+    std::printf("meinv: %s: box %dx%d (%" PRId64 " px), inverted %" PRId64 " px in %" PRId64 " chunks, valid %" PRId64
                 ", converged %" PRId64 " (chi2 <= %g); %.2f s wall, %.1f ms kernel%s\n",
-                nxb, nyb, counts, inverted, done_chunks, valid, converged, ctl.cfg.chisq_criteria, dt, kernel_ms,
-                resume ? " (resumed)" : "");
+                pa.data.c_str(), nxb, nyb, counts, inverted, done_chunks, valid, converged, ctl.cfg.chisq_criteria,
+                dt, kernel_ms, resume ? " (resumed)" : "");
+    std::fflush(stdout);
     return 0;
+}
+
+std::string stem_of(const std::string &path)
+{
+    size_t slash = path.find_last_of('/');
+    std::string base = slash == std::string::npos ? path : path.substr(slash + 1);
+    size_t dot = base.rfind('.');
+    return dot == std::string::npos ? base : base.substr(0, dot);
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    std::string control = "./input";
+    Options opt;
+    bool have_control = false;
+    std::vector<std::string> files;
+    std::string result_dir;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "--dump-config") return dump_config(i + 1 < argc ? argv[i + 1] : "./input");
+        if (a == "--fits-info" && i + 1 < argc) return fits_info(argv[i + 1]);
+        if (a == "--fits-demo" && i + 1 < argc) return fits_demo(argv[i + 1]);
+        if (a == "--fits-read" && i + 3 < argc) return fits_read(argv[i + 1], std::atoll(argv[i + 2]), std::atoll(argv[i + 3]));
+        if (a == "--precision" && i + 1 < argc) {
+            std::string p = argv[++i];
+            if (p == "fp32") opt.prec = ME_FP32;
+            else if (p == "fp64") opt.prec = ME_FP64;
+            else return fail("--precision fp64|fp32");
+        } else if (a == "--batch" && i + 1 < argc) {
+            opt.batch = std::atoll(argv[++i]);
+            if (opt.batch < 1) return fail("--batch N needs N >= 1");
+        } else if (a == "--max-chunks" && i + 1 < argc) {
+            opt.max_chunks = std::atoll(argv[++i]);
+        } else if (a == "--result-dir" && i + 1 < argc) {
+            result_dir = argv[++i];
+        } else if (a == "--files") {
+            while (i + 1 < argc && argv[i + 1][0] != '-') files.push_back(argv[++i]);
+        } else if (!a.empty() && a[0] != '-' && !have_control) {
+            control = a;
+            have_control = true;
+        } else {
+            return fail("unknown argument " + a);
+        }
+    }
+
+    mehost::Control ctl;
+    std::string err;
+    if (!mehost::parse_control(control, ctl, err)) return fail(err);
+    for (const std::string &w : ctl.warnings)
+        if (ctl.verbose > 0) std::fprintf(stderr, "meinv: %s\n", w.c_str());
+
+    // the wavelength grid (Src/IO.c:120-286): one context serves every cube
+    mefits::Image wav;
+    if (!mefits::open_image(ctl.wavelength_path, wav, err)) return fail(err);
+    if (wav.naxes.size() != 1) return fail("error in the dimension of wavelength hdu.");
+    if (wav.bitpix != -32 && wav.bitpix != -64) return fail("error in the data type of the wavelength hdu.");
+    const int nl = (int)wav.naxes[0];
+    std::vector<double> lambda((size_t)nl);
+    if (!mefits::read_pixels(wav, 0, nl, lambda.data(), err)) return fail(err);
+
+    me_config cfg = ctl.cfg;
+    cfg.precision = opt.prec;
+    me_ctx *ctx = nullptr;
+    me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
+    if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");
+
+    int rc = 0;
+    if (files.empty()) {
+        rc = invert_cube(ctl, opt, Paths{ctl.data_path, ctl.result_path, ctl.fit_path, ctl.cache_path}, lambda, ctx);
+    } else {
+        // several cubes on the same grid (the loop of Tools/RunMEINV.py, in one
+        // process): <result-dir>/<stem>_res.fits, _fit.fits and _cache
+        const std::string dir = result_dir.empty() ? "." : result_dir;
+        for (const std::string &f : files) {
+            const std::string stem = dir + "/" + stem_of(f);
+            rc = invert_cube(ctl, opt, Paths{f, stem + "_res.fits", stem + "_fit.fits", stem + "_cache"}, lambda, ctx);
+            if (rc) break;
+        }
+    }
+    me_ctx_destroy(ctx);
+    return rc;
 }

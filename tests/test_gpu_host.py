@@ -98,3 +98,31 @@ def test_host_interrupted_run_resumes_from_the_cache(tmp_path):
     np.testing.assert_array_equal(res.reshape(-1, 10), ref["res"])
     flags = np.frombuffer((tmp_path / "cache").read_bytes()[16:], dtype="<i4")
     assert flags[:-1].all()
+
+
+def test_batch_driver_inverts_every_cube_with_one_context(tmp_path, monkeypatch):
+    """meinv.batch (Tools/RunMEINV.py's command line): two cubes in a
+    directory, one host process, res/<stem>_res.fits each equal to the library
+    on that cube."""
+    from meinv import batch
+    w, orc, prof, truth = oracle_map("niris64", 2 * 300)
+    data = tmp_path / "data"
+    data.mkdir()
+    cubes = []
+    for k in range(2):
+        cube = prof[300 * k:300 * (k + 1)].reshape(15, 20, 4, w.nl).astype(np.float32)
+        _write_fits(data / f"scan{k}.fits", cube, -32)
+        cubes.append(cube)
+    _write_fits(tmp_path / "wav.fits", w.wavelengths(), -64)
+    settings = dict(batch.SETTINGS, wavelength_path=str(tmp_path / "wav.fits"), lines=f"{w.lambda0}, {w.geff}",
+                    num_run=1, Icriteria=0.1, chisq_criteria=1.0, nprofiles=64)
+    monkeypatch.chdir(tmp_path)
+    outs = batch.run(data, settings=settings)
+    text = (tmp_path / "input").read_text()
+    cfg = to_config(parse_text(text))
+    for k, out in enumerate(outs):
+        assert out == data / "res" / f"scan{k}_res.fits"
+        _, res = _read_fits(out)
+        with Inversion(cfg, w.wavelengths()) as inv:
+            ref = inv.invert(cubes[k].reshape(-1, 4, w.nl).astype(np.float64))
+        np.testing.assert_array_equal(res.reshape(-1, 10), ref["res"])
