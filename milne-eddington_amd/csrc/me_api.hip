@@ -22,7 +22,7 @@ struct DevBuf {
     hipError_t ensure(size_t b)
     {
         if (b <= bytes) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);  // the buffer is replaced either way
         p = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&p, b);
@@ -31,7 +31,7 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
@@ -42,7 +42,8 @@ struct me_ctx {
     int nl = 0, device = 0, ncu = 0;
     DevCfg dc;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // inversion kernel
+    hipEvent_t et0 = nullptr, et1 = nullptr;  // whole call (staging included)
     double *d_lam = nullptr, *d_dlam = nullptr, *d_etab = nullptr;
     float *d_dlamf = nullptr, *d_etabf = nullptr;
     unsigned long long *d_queue = nullptr, *d_counters = nullptr;
@@ -97,8 +98,7 @@ me_status hip_status(hipError_t e)
 double ms_between(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, a, b);
-    return (double)ms;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? (double)ms : NAN;
 }
 
 // grid of the persistent inversion kernel: every resident slot of the chip,
@@ -108,8 +108,8 @@ void persistent_grid(const me_ctx *c, K kernel, int64_t n, int &block, int &grid
 {
     block = c->cfg.block_size > 0 ? c->cfg.block_size : 256;
     int per_cu = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
-    if (per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;  // one resident block per CU is always possible for this kernel
     int waves_cu = per_cu * (block / 64);
     if (c->cfg.waves_per_cu > 0) waves_cu = std::min(waves_cu, c->cfg.waves_per_cu);
     int blocks_cu = std::max(1, waves_cu / (block / 64));
@@ -199,10 +199,7 @@ me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double
         st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
     }
     if (n == 0) return ME_OK;
-    hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
-    hipEventRecord(t0, c->stream);
+    HIPCHK(hipEventRecord(c->et0, c->stream));
     const int nl = c->nl;
     HIPCHK(c->prof.ensure(esz * 4 * nl * n));
     HIPCHK(c->res.ensure(sizeof(double) * 10 * n));
@@ -223,16 +220,14 @@ me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double
         px.resize(4 * n);
         HIPCHK(hipMemcpyAsync(px.data(), c->pxs.p, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, c->stream));
     }
-    hipEventRecord(t1, c->stream);
+    HIPCHK(hipEventRecord(c->et1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (st) {
-        st->total_ms = ms_between(t0, t1);
+        st->total_ms = ms_between(c->et0, c->et1);
         int32_t *dst[4] = {st->iters, st->runs, st->trials, st->status};
         for (int k = 0; k < 4; k++)
             if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
     return ME_OK;
 }
 
@@ -300,9 +295,11 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     me_ctx *c = new me_ctx();
     c->cfg = *cfg;
     c->nl = nl;
-    hipGetDevice(&c->device);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) { delete c; return ME_EDEVICE; }
+    if (hipGetDevice(&c->device) != hipSuccess || hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+        delete c;
+        return ME_EDEVICE;
+    }
     c->ncu = prop.multiProcessorCount;
 
     DevCfg &d = c->dc;
@@ -383,6 +380,8 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     e = e ? e : hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     e = e ? e : hipEventCreate(&c->ev0);
     e = e ? e : hipEventCreate(&c->ev1);
+    e = e ? e : hipEventCreate(&c->et0);
+    e = e ? e : hipEventCreate(&c->et1);
     e = e ? e : hipMalloc(&c->d_lam, sizeof(double) * nl);
     e = e ? e : hipMalloc(&c->d_dlam, sizeof(double) * nl);
     e = e ? e : hipMalloc(&c->d_dlamf, sizeof(float) * nl);
@@ -418,21 +417,18 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
 void me_ctx_destroy(me_ctx *c)
 {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->d_lam);
-    hipFree(c->d_dlam);
-    hipFree(c->d_dlamf);
-    hipFree(c->d_etab);
-    hipFree(c->d_etabf);
-    hipFree(c->d_queue);
-    hipFree(c->d_counters);
-    hipFree(c->d_cfg);
+    // teardown: a failing free leaves nothing to recover, so results are not checked
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *owned[] = {c->d_lam, c->d_dlam, c->d_dlamf, c->d_etab, c->d_etabf, c->d_queue, c->d_counters, c->d_cfg};
+    for (void *p : owned)
+        if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar};
     for (DevBuf *b : bufs) b->release();
     me::mlp_free(c->mlp);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->et0, c->et1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -470,10 +466,7 @@ me_status me_invert_device(me_ctx *c, const void *d_prof, int prof_is_f64, int64
         st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
     }
     if (n == 0) return ME_OK;
-    hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
-    hipEventRecord(t0, c->stream);
+    HIPCHK(hipEventRecord(c->et0, c->stream));
     const double *dinit = d_init;
     if (learned) {
         if (!c->mlp.trained) return ME_ENOTREADY;
@@ -498,11 +491,9 @@ me_status me_invert_device(me_ctx *c, const void *d_prof, int prof_is_f64, int64
         for (int k = 0; k < 4; k++)
             if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
     }
-    hipEventRecord(t1, c->stream);
+    HIPCHK(hipEventRecord(c->et1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (st) st->total_ms = ms_between(t0, t1);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    if (st) st->total_ms = ms_between(c->et0, c->et1);
     return ME_OK;
 }
 
@@ -543,8 +534,9 @@ me_status me_gen_map_device(me_ctx *c, uint64_t seed, const double *ranges, doub
     hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, c->dc, seed, (const double *)d_rg,
                        ff_frac, ff_bmax, sigma, n, d_prof, d_truth);
     hipError_t e = hipGetLastError();
-    hipFreeAsync(d_rg, c->stream);
+    hipError_t ef = hipFreeAsync(d_rg, c->stream);
     HIPCHK(e);
+    HIPCHK(ef);
     HIPCHK(hipStreamSynchronize(c->stream));
     return ME_OK;
 }
@@ -566,8 +558,9 @@ me_status me_mlp_train(me_ctx *c, uint64_t seed, const double *ranges, double ff
     };
     me_status s = me::mlp_train_impl(c->mlp, c->dc, c->stream, seed, ranges, n_train, batch, epochs, lr, final_loss,
                                      gen);
-    hipStreamSynchronize(c->stream);
-    hipFree(d_rg);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    const hipError_t ef = hipFree(d_rg);
+    if (s == ME_OK && (es != hipSuccess || ef != hipSuccess)) s = ME_EDEVICE;
     return s;
 }
 

@@ -462,6 +462,14 @@ struct MlpState {
     int packed = 0;
 };
 
+// free a device buffer and forget it (a failing free at this point leaves
+// nothing to recover)
+template <typename T> inline void dev_release(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
 inline int64_t mlp_num_params(const MlpState &m)
 {
     // exported layout: only the real input rows (nin) and output columns (10)
@@ -471,10 +479,10 @@ inline int64_t mlp_num_params(const MlpState &m)
 
 inline void mlp_free(MlpState &m)
 {
-    hipFree(m.P); hipFree(m.Mo); hipFree(m.Vo); hipFree(m.G); hipFree(m.Pb);
-    hipFree(m.X); hipFree(m.Y); hipFree(m.loss); hipFree(m.prof_tmp); hipFree(m.truth_tmp);
-    hipFree(m.wpk); hipFree(m.bpk);
-    for (int l = 0; l < MLP_NL; l++) { hipFree(m.A[l]); hipFree(m.D[l]); hipFree(m.Z[l]); }
+    dev_release(m.P); dev_release(m.Mo); dev_release(m.Vo); dev_release(m.G); dev_release(m.Pb);
+    dev_release(m.X); dev_release(m.Y); dev_release(m.loss); dev_release(m.prof_tmp); dev_release(m.truth_tmp);
+    dev_release(m.wpk); dev_release(m.bpk);
+    for (int l = 0; l < MLP_NL; l++) { dev_release(m.A[l]); dev_release(m.D[l]); dev_release(m.Z[l]); }
     m = MlpState();
 }
 
@@ -508,8 +516,9 @@ inline me_status mlp_alloc_params(MlpState &m)
 inline me_status mlp_alloc_acts(MlpState &m, int64_t rows)
 {
     if (rows <= m.cap) return ME_OK;
-    hipFree(m.X); hipFree(m.Y);
-    for (int l = 0; l < MLP_NL; l++) { hipFree(m.A[l]); hipFree(m.D[l]); hipFree(m.Z[l]); }
+    m.cap = 0;
+    dev_release(m.X); dev_release(m.Y);
+    for (int l = 0; l < MLP_NL; l++) { dev_release(m.A[l]); dev_release(m.D[l]); dev_release(m.Z[l]); }
     hipError_t e = hipMalloc(&m.X, sizeof(__bf16) * rows * m.ninp);
     e = e ? e : hipMalloc(&m.Y, sizeof(float) * rows * MLP_OUTP);
     for (int l = 0; l < MLP_NL; l++) {
@@ -600,8 +609,9 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
 inline me_status mlp_tmp(MlpState &m, int64_t rows, int nl)
 {
     if (rows <= m.tmp_cap) return ME_OK;
-    hipFree(m.prof_tmp);
-    hipFree(m.truth_tmp);
+    m.tmp_cap = 0;
+    dev_release(m.prof_tmp);
+    dev_release(m.truth_tmp);
     hipError_t e = hipMalloc(&m.prof_tmp, sizeof(float) * rows * 4 * nl);
     e = e ? e : hipMalloc(&m.truth_tmp, sizeof(double) * rows * 9);
     if (e != hipSuccess) return ME_ENOMEM;
@@ -653,8 +663,9 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
                            m.Pb, m.np, lrs, b1, b2, 1e-8f, c1, c2);
         if (step == total - 1) {
             float lh = 0;
-            hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
-            hipStreamSynchronize(s);
+            if (hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return ME_EDEVICE;
             lossv = lh;
         }
     }
@@ -696,8 +707,9 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
             m.packed = 1;
         }
         int dev = 0, ncu = 1;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return ME_EDEVICE;
         const int64_t passes = (n + MI_PX - 1) / MI_PX;
         const unsigned grid = (unsigned)(passes < ncu ? passes : ncu);
         if (grid == 0) return ME_OK;
