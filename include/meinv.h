@@ -124,6 +124,13 @@ void      me_ctx_destroy(me_ctx *ctx);
 /* Derived constants of the context: ic, i0, i1, ncut and v_los bounds. */
 me_status me_ctx_info(const me_ctx *ctx, int *ic, int *i0, int *i1, int *ncut, double vlos_bounds[2]);
 
+/* Whether ME_FP32 inversions classify every Faddeeva call by the region the
+ * FP64 evaluation would use and count it in me_stats.n_fadd (on after
+ * me_ctx_create).  The classification costs a few per cent of kernel time;
+ * results do not depend on it.  No counterpart in the reference: the counts
+ * feed the benchmark's operation model. */
+me_status me_ctx_set_region_counts(me_ctx *ctx, int on);
+
 /*
  * Invert n pixels.  Replaces INVERSION_MULTI (Src/LM_FIT.c:746-811,
  * declared Src/LM_FIT.h:43-44) together with the MPI pixel scatter of

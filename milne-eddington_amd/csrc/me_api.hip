@@ -363,6 +363,7 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     int ifig = std::min(4, cfg->nfigures > 4 ? cfg->nfigures - 4 : 0);
     for (int k = 0; k < 5; k++) d.fthr[k] = kThr[ifig][k];
     weideman_coefficients(16, d.wL, d.wc);
+    d.fregions = 1;
     d.seed = cfg->seed;
 
     double etab[50];
@@ -430,6 +431,15 @@ void me_ctx_destroy(me_ctx *c)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+me_status me_ctx_set_region_counts(me_ctx *c, int on)
+{
+    if (!c) return ME_EINVAL;
+    c->dc.fregions = on ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_cfg, &c->dc, sizeof(DevCfg), hipMemcpyHostToDevice));
+    return ME_OK;
 }
 
 me_status me_ctx_info(const me_ctx *c, int *ic, int *i0, int *i1, int *ncut, double vlos[2])

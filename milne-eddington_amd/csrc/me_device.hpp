@@ -41,6 +41,7 @@ struct DevCfg {
     double fa, faa, flogr, fsqrtlogr;  // Algorithm 916 constants
     double fthr[5];                    // |z|^2 region thresholds for nfigures
     float wL, wc[16];                  // FP32 path: Weideman N = 16 rational approximation
+    int fregions;                      // FP32 path: count each call by its FP64 region
     uint64_t seed;
     const double *lam;    // [nl] wavelengths [Å]
     const double *dlam;   // [nl] lambda - lambda0, FP64
@@ -464,14 +465,13 @@ __device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T st
         const int k = q + 1;
         const T v = shW + T(q) * K.shB;
         T H, L;
-        int b;
         if constexpr (std::is_same<T, float>::value) {
             weideman16(v, K.damp, C, H, L);
-            b = fcnt ? faddeeva_branch<T>(v, K.damp, C) : 0;
+            if (fcnt && C.fregions) fcnt->add(faddeeva_branch<T>(v, K.damp, C));
         } else {
-            b = faddeeva<T>(v, K.damp, C, H, L);
+            const int b = faddeeva<T>(v, K.damp, C, H, L);
+            if (fcnt) fcnt->add(b);
         }
-        if (fcnt) fcnt->add(b);
         ph[k] = H / SP;
         ps[k] = L / SP;
         if (JAC) {
