@@ -296,8 +296,8 @@ struct Sol9 {
     int status, nbk;
 };
 
-__device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
-                                       const double *__restrict__ gv, int64_t NS, double thr0, int npar)
+__device__ __forceinline__ Sol9 svd_solve_body(const double *__restrict__ in, double *__restrict__ uo,
+                                               const double *__restrict__ gv, int64_t NS, double thr0, int npar)
 {
     Sol9 r;
     double M[81], w[9];
@@ -366,6 +366,15 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
         thr *= 5.;
     }
     return r;
+}
+
+// The call boundary sits on a thin wrapper: with the body inlined into it the
+// allocator keeps the SVD in 256 VGPRs + 160 AGPRs and 144 B of spill per
+// lane, against 256 + 256 and 300 B when the body is the call target itself.
+__device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
+                                       const double *__restrict__ gv, int64_t NS, double thr0, int npar)
+{
+    return svd_solve_body(in, uo, gv, NS, thr0, npar);
 }
 
 // Fixed parameters (Src/LM_FIT.c:316-347): keep the rows and columns of the
