@@ -802,65 +802,88 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         }
         a[i * 9 + i] += 1.0;
     });
-    // 4. implicit-shift QR on the bidiagonal form.  k is wave-uniform; the
-    //    split index L differs per lane and is only used through selects.
+    // 4. implicit-shift QR on the bidiagonal form.  Every lane runs its own
+    //    sequence of NR steps: its own current singular value k, split index L
+    //    and sweep count; one pass of the outer loop is one step for each lane
+    //    that is not done.  A lane whose k has converged moves on to k - 1 at
+    //    once instead of waiting for the slowest lane of the wave, so the wave
+    //    runs max(sum of steps) rather than sum over k of max(steps at k).
+    //    Indices that depend on k or L are only used through selects.
     int fail = 0;
+    int k = 8, its = 0;
 #pragma unroll 1
-    for (int k = 8; k >= 0; k--) {
-#pragma unroll 1
-        for (int its = 0; its < 50; its++) {
-            int L = 0;
-            bool cancel = false, found = false;
+    for (int guard = 0; guard < 9 * 50 + 64; guard++) {
+        // negligible e[l] (bit l of nme) and w[l-1] (bit l of nmw); e[0] is 0
+        unsigned nme = 0, nmw = 0;
 #pragma unroll
-            for (int ll = 8; ll >= 0; ll--) {
-                if (ll <= k && !found) {
-                    if (fabs(e[ll]) + anorm == anorm) { L = ll; cancel = false; found = true; }
-                    else if (ll >= 1 && fabs(w[ll - 1]) + anorm == anorm) { L = ll; cancel = true; found = true; }
-                }
+        for (int ll = 0; ll < 9; ll++) {
+            nme |= (fabs(e[ll]) + anorm == anorm) ? (1u << ll) : 0u;
+            nmw |= (fabs(w[ll]) + anorm == anorm) ? (2u << ll) : 0u;
+        }
+        // split search from l = k down (e[l] tested before w[l-1]); lanes
+        // whose k converged without a cancellation take the next k
+        int L = 0;
+        bool cancel = false;
+#pragma unroll 1
+        for (int t = 0; t < 10; t++) {
+            const unsigned cand = (nme | nmw) & ((2u << (k < 0 ? 0 : k)) - 1u);
+            L = 31 - __builtin_clz(cand | 1u);
+            cancel = ((nme >> L) & 1u) == 0u;
+            const bool conv = k >= 0 && L == k && !cancel;
+            if (__ballot(conv) == 0) break;
+            if (conv) {
+#pragma unroll
+                for (int kk = 0; kk < 9; kk++)
+                    if (kk == k && w[kk] < 0.0) w[kk] = -w[kk];
+                k--;
+                its = 0;
             }
-            if (cancel) {
-                // rotate column L-1 against columns L..k of U
-                double col[9];
+        }
+        if (__ballot(k >= 0) == 0) break;
+        if (k >= 0 && cancel) {
+            // rotate column L-1 against columns L..k of U
+            double col[9];
 #pragma unroll
-                for (int r = 0; r < 9; r++) col[r] = 0.0;
+            for (int r = 0; r < 9; r++) col[r] = 0.0;
 #pragma unroll
-                for (int cc = 0; cc < 8; cc++)
-                    if (cc == L - 1) {
+            for (int cc = 0; cc < 8; cc++)
+                if (cc == L - 1) {
 #pragma unroll
-                        for (int r = 0; r < 9; r++) col[r] = a[r * 9 + cc];
-                    }
-                double c = 0.0, s = 1.0;
-                bool stop = false;
+                    for (int r = 0; r < 9; r++) col[r] = a[r * 9 + cc];
+                }
+            double c = 0.0, s = 1.0;
+            bool stop = false;
 #pragma unroll
-                for (int i = 1; i < 9; i++) {
-                    if (!stop && i >= L && i <= k) {
-                        double f = s * e[i];
-                        e[i] = c * e[i];
-                        if (fabs(f) + anorm == anorm) {
-                            stop = true;
-                        } else {
-                            double gg = w[i];
-                            double h, rh;
-                            hyp_r(f, gg, h, rh);
-                            w[i] = h;
-                            c = gg * rh;
-                            s = -f * rh;
+            for (int i = 1; i < 9; i++) {
+                if (!stop && i >= L && i <= k) {
+                    double f = s * e[i];
+                    e[i] = c * e[i];
+                    if (fabs(f) + anorm == anorm) {
+                        stop = true;
+                    } else {
+                        double gg = w[i];
+                        double h, rh;
+                        hyp_r(f, gg, h, rh);
+                        w[i] = h;
+                        c = gg * rh;
+                        s = -f * rh;
 #pragma unroll
-                            for (int r = 0; r < 9; r++) {
-                                double yv = col[r], zv = a[r * 9 + i];
-                                col[r] = yv * c + zv * s;
-                                a[r * 9 + i] = zv * c - yv * s;
-                            }
+                        for (int r = 0; r < 9; r++) {
+                            double yv = col[r], zv = a[r * 9 + i];
+                            col[r] = yv * c + zv * s;
+                            a[r * 9 + i] = zv * c - yv * s;
                         }
                     }
                 }
-#pragma unroll
-                for (int cc = 0; cc < 8; cc++)
-                    if (cc == L - 1) {
-#pragma unroll
-                        for (int r = 0; r < 9; r++) a[r * 9 + cc] = col[r];
-                    }
             }
+#pragma unroll
+            for (int cc = 0; cc < 8; cc++)
+                if (cc == L - 1) {
+#pragma unroll
+                    for (int r = 0; r < 9; r++) a[r * 9 + cc] = col[r];
+                }
+        }
+        if (k >= 0) {
             double z = 0.0, wkm1 = 0.0, ek = 0.0, ekm1 = 0.0, x = 0.0;
 #pragma unroll
             for (int kk = 0; kk < 9; kk++) {
@@ -869,61 +892,69 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 if (kk == L) x = w[kk];
             }
             if (L == k) {
+                // converged right after a cancellation
                 if (z < 0.0) {
 #pragma unroll
                     for (int kk = 0; kk < 9; kk++)
                         if (kk == k) w[kk] = -z;
                 }
-                break;
-            }
-            if (its == 49) fail = 1;
-            double yv = wkm1, gg = ekm1, h = ek;
-            double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) / (2.0 * h * yv);
-            gg = hyp(f, 1.0);
-            double sg = f >= 0.0 ? gg : -gg;
-            f = ((x - z) * (x + z) + h * (yv / (f + sg) - h)) / x;
-            double c = 1.0, s = 1.0;
+                k--;
+                its = 0;
+            } else {
+                if (its == 49) fail = 1;
+                its++;
+                double yv = wkm1, gg = ekm1, h = ek;
+                double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) / (2.0 * h * yv);
+                gg = hyp(f, 1.0);
+                double sg = f >= 0.0 ? gg : -gg;
+                f = ((x - z) * (x + z) + h * (yv / (f + sg) - h)) / x;
+                double c = 1.0, s = 1.0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                if (j < k && j >= L) {
-                    const int i = j + 1;
-                    gg = e[i];
-                    yv = w[i];
-                    h = s * gg;
-                    gg *= c;
-                    double rz;
-                    hyp_r(f, h, z, rz);
-                    e[j] = z;
-                    c = f * rz;
-                    s = h * rz;
-                    f = x * c + gg * s;
-                    gg = gg * c - x * s;
-                    h = yv * s;
-                    yv *= c;
-                    hyp_r(f, h, z, rz);
-                    w[j] = z;
-                    if (z != 0.0) {
+                for (int j = 0; j < 8; j++) {
+                    if (j < k && j >= L) {
+                        const int i = j + 1;
+                        gg = e[i];
+                        yv = w[i];
+                        h = s * gg;
+                        gg *= c;
+                        double rz;
+                        hyp_r(f, h, z, rz);
+                        e[j] = z;
                         c = f * rz;
                         s = h * rz;
-                    }
-                    f = c * gg + s * yv;
-                    x = c * yv - s * gg;
+                        f = x * c + gg * s;
+                        gg = gg * c - x * s;
+                        h = yv * s;
+                        yv *= c;
+                        hyp_r(f, h, z, rz);
+                        w[j] = z;
+                        if (z != 0.0) {
+                            c = f * rz;
+                            s = h * rz;
+                        }
+                        f = c * gg + s * yv;
+                        x = c * yv - s * gg;
 #pragma unroll
-                    for (int r = 0; r < 9; r++) {
-                        double y2 = a[r * 9 + j], z2 = a[r * 9 + i];
-                        a[r * 9 + j] = y2 * c + z2 * s;
-                        a[r * 9 + i] = z2 * c - y2 * s;
+                        for (int r = 0; r < 9; r++) {
+                            double y2 = a[r * 9 + j], z2 = a[r * 9 + i];
+                            a[r * 9 + j] = y2 * c + z2 * s;
+                            a[r * 9 + i] = z2 * c - y2 * s;
+                        }
                     }
                 }
-                SVD_BARRIER();
-            }
 #pragma unroll
-            for (int kk = 0; kk < 9; kk++) {
-                if (kk == L) e[kk] = 0.0;
-            }
+                for (int kk = 0; kk < 9; kk++) {
+                    if (kk == L) e[kk] = 0.0;
+                }
 #pragma unroll
-            for (int kk = 0; kk < 9; kk++) {
-                if (kk == k) { e[kk] = f; w[kk] = x; }
+                for (int kk = 0; kk < 9; kk++) {
+                    if (kk == k) { e[kk] = f; w[kk] = x; }
+                }
+                // 50 sweeps without convergence: flagged, go on with k - 1
+                if (its >= 50) {
+                    k--;
+                    its = 0;
+                }
             }
         }
     }
