@@ -21,11 +21,12 @@ enum {
     CT_RUNS, CT_ITERS, CT_VALID, CT_CONV, CT_SVDFAIL, CT_FALLBACK, CT_N
 };
 
-// per-lane slots in the scratch buffer (structure of arrays)
-// SL_MAT holds two 81-entry matrices: the trial's input A and its left
-// factor U alternate between them (mbuf selects the input)
-enum { SL_BEST = 0, SL_GUESS = 9, SL_LEARN = 18, SL_CHI_BEST = 27, SL_CHI_LEARN = 28, SL_G = 29, SL_MAT = 40,
-       SL_N = 40 + 162 };
+// per-lane slots in LDS (one column per lane of the block)
+enum { SL_BEST = 0, SL_GUESS = 9, SL_LEARN = 18, SL_CHI_BEST = 27, SL_CHI_LEARN = 28, SL_LDS_N = 29 };
+// per-lane slots in the scratch buffer (structure of arrays): the gradient,
+// then two 81-entry matrices: the trial's input A and its left factor U
+// alternate between them (mbuf selects the input)
+enum { SL_G = 0, SL_MAT = 9, SL_N = 9 + 162 };
 
 struct InvArgs {
     const void *prof;
@@ -404,7 +405,12 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *scr = A.scratch + slot;
     const int64_t NS = A.nslots;
-#define SCR(k) scr[(int64_t)(k) * NS]
+    // the small per-lane state (best, guess and learned parameters, their
+    // chi2) lives in LDS, one column per lane; the gradient and the matrices
+    // stay in the scratch buffer
+    __shared__ double lsl[SL_LDS_N][256];
+    double *mysl = &lsl[0][threadIdx.x];
+#define SCR(k) mysl[(k) * 256]
     // event counters live in LDS, one column per lane
     __shared__ uint32_t lct[CT_N][256];
     uint32_t *myct = &lct[0][threadIdx.x];
@@ -431,6 +437,11 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             if (p >= A.n) { phase = PH_DONE; continue; }
             pix = p;
             const int64_t base = pix * 4 * nl;
+            // explicit or learned initial parameters, loaded together with
+            // the continuum so that both arrive in one round trip
+            double ini[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) ini[k] = A.init ? A.init[pix * 9 + k] : 0.0;
             double mean = 0.0;
             for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(A.prof, base + iw);
             mean /= nl;
@@ -460,7 +471,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             normp = G.normp;
             if (A.init && !A.learned) {
 #pragma unroll
-                for (int k = 0; k < 9; k++) gs[k] = A.init[pix * 9 + k];
+                for (int k = 0; k < 9; k++) gs[k] = ini[k];
             }
             bounds_check(C, gs, l7lo, l7hi);
 #pragma unroll
@@ -470,7 +481,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             }
             if (A.learned) {
 #pragma unroll
-                for (int k = 0; k < 9; k++) par[k] = A.init[pix * 9 + k];
+                for (int k = 0; k < 9; k++) par[k] = ini[k];
                 bounds_check(C, par, l7lo, l7hi);
                 lphase = 1;
             } else {
