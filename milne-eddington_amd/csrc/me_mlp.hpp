@@ -35,7 +35,8 @@ __device__ __forceinline__ float from_bf16(__bf16 x) { return __bfloat162float(x
 __device__ __forceinline__ float erf_as(float x)
 {
     const float ax = fabsf(x);
-    const float t = 1.0f / (1.0f + 0.3275911f * ax);
+    // 1 + p|x| >= 1: the hardware reciprocal needs no range scaling here
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
     const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
     const float y = 1.0f - poly * __expf(-ax * ax);
     return copysignf(y, x);
@@ -323,6 +324,47 @@ __global__ void mlp_pack_kernel(const __bf16 *__restrict__ Pb, const float *__re
     wpk[f] = v;
 }
 
+// one hidden layer (16 output tiles of 16 channels, GELU) for the wave's
+// MI_C pixel tiles; KB (k blocks of 32) is a compile-time constant so the
+// MFMA chain of a tile has no branches between its steps
+template <int KB>
+__device__ __forceinline__ void mlp_hidden_layer(const bf16x8 *__restrict__ sW, const float *__restrict__ sb, int lane,
+                                                 int g, bf16x8 (&act)[MI_C][8])
+{
+    bf16x8 nxt[MI_C][8];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        // one output tile at a time: keeps the scheduler from hoisting every
+        // tile's LDS reads into registers
+        __builtin_amdgcn_sched_barrier(0);
+        f32x4 acc[MI_C];
+#pragma unroll
+        for (int c = 0; c < MI_C; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            const bf16x8 a = sW[(t * KB + kb) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < MI_C; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, act[c][kb], acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < MI_C; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) nxt[c][t >> 1][4 * (t & 1) + v] = to_bf16(gelu(acc[c][v] + sb[16 * t + 4 * g + v]));
+    }
+#pragma unroll
+    for (int c = 0; c < MI_C; c++)
+#pragma unroll
+        for (int kb = 0; kb < 8; kb++) act[c][kb] = nxt[c][kb];
+}
+
+// k blocks of the first layer: 4 nl inputs in blocks of 32, rounded up to a
+// power of two (the packed weights are zero beyond the inputs)
+inline int mlp_kb0_padded(int ninp)
+{
+    const int kb = ninp / 32;
+    return kb <= 1 ? 1 : (kb <= 2 ? 2 : (kb <= 4 ? 4 : 8));
+}
+
 template <typename TIn>
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const TIn *__restrict__ prof, int64_t n, int nl, int kb0,
                                                         const bf16x8 *__restrict__ wpk, MlpPack d,
@@ -370,34 +412,10 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const TIn *__restrict__ 
             sb[tid] = bpk[l * 256 + tid];
             __syncthreads();
             if (l < MLP_NL - 1) {
-                bf16x8 nxt[MI_C][8];
-#pragma unroll
-                for (int t = 0; t < 16; t++) {
-                    // one output tile at a time: keeps the scheduler from
-                    // hoisting every tile's LDS reads into registers
-                    __builtin_amdgcn_sched_barrier(0);
-                    f32x4 acc[MI_C];
-#pragma unroll
-                    for (int c = 0; c < MI_C; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int kb = 0; kb < 8; kb++) {
-                        if (kb < KB) {
-                            const bf16x8 a = sW[(t * KB + kb) * 64 + lane];
-#pragma unroll
-                            for (int c = 0; c < MI_C; c++)
-                                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, act[c][kb], acc[c], 0, 0, 0);
-                        }
-                    }
-#pragma unroll
-                    for (int c = 0; c < MI_C; c++)
-#pragma unroll
-                        for (int v = 0; v < 4; v++)
-                            nxt[c][t >> 1][4 * (t & 1) + v] = to_bf16(gelu(acc[c][v] + sb[16 * t + 4 * g + v]));
-                }
-#pragma unroll
-                for (int c = 0; c < MI_C; c++)
-#pragma unroll
-                    for (int kb = 0; kb < 8; kb++) act[c][kb] = nxt[c][kb];
+                if (KB == 8) mlp_hidden_layer<8>(sW, sb, lane, g, act);
+                else if (KB == 4) mlp_hidden_layer<4>(sW, sb, lane, g, act);
+                else if (KB == 2) mlp_hidden_layer<2>(sW, sb, lane, g, act);
+                else mlp_hidden_layer<1>(sW, sb, lane, g, act);
             } else {
                 // output layer: 16 channels (10 used); lane (g, col) holds
                 // channels 4g..4g+3 of pixel col; lanes of group 0 gather
@@ -692,7 +710,7 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
                 d.din[l] = m.dims[l];
                 d.dout[l] = m.dims[l + 1];
                 d.T[l] = m.dims[l + 1] / 16;
-                d.KB[l] = m.dims[l] / 32;
+                d.KB[l] = l == 0 ? mlp_kb0_padded(m.dims[0]) : m.dims[l] / 32;
                 d.frag[l] = fr;
                 fr += (int64_t)d.T[l] * d.KB[l];
             }
@@ -715,10 +733,10 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
         if (grid == 0) return ME_OK;
         if (prof_is_f64)
             hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(grid), dim3(256), 0, s, (const double *)prof, n, nl,
-                               m.pack.KB[0], (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
+                               m.ninp / 32, (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
         else
             hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(grid), dim3(256), 0, s, (const float *)prof, n, nl,
-                               m.pack.KB[0], (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
+                               m.ninp / 32, (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
         return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
     }
     const int64_t chunk = 1 << 20;
