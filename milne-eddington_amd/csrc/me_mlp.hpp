@@ -282,7 +282,7 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__
 // consecutive output channels per 16-channel tile; the next layer reads its
 // k index in that order, so the packed weights of layers 1..3 permute their
 // input rows to match (mlp_in_channel) and no data moves between lanes.
-constexpr int MI_C = 2;                  // pixel tiles (16 each) per wave
+constexpr int MI_C = 4;                  // pixel tiles (16 each) per wave
 constexpr int MI_PX = 4 * 16 * MI_C;     // pixels per block pass
 
 // input channel that k element q of k-block kb, lane group g reads
