@@ -1,11 +1,13 @@
 """Learned initialiser on the device: BF16-MFMA gradients against the CPU
 oracle, and what a trained network gains over plain random-restart LM."""
+import dataclasses
+
 import numpy as np
 import pytest
 
 import mlp_oracle as M
 from gpu_util import device_tensor, gpu_inversion, oracle_map
-from meinv import ME_FP32, ME_INIT_LEARNED
+from meinv import ME_FP32, ME_INIT_LEARNED, Inversion
 from meinv.workloads import get
 
 pytestmark = pytest.mark.gpu
@@ -116,16 +118,25 @@ def _numpy_predict(params, prof, nl, ranges):
     return out, z
 
 
-@pytest.mark.parametrize("name", ["hmi4k", "niris64", "hinode"])
+def _workload(name):
+    """A named workload, or the hmi4k line on an nl-sample grid ("hmi4k-nl12")."""
+    if name.startswith("hmi4k-nl"):
+        return dataclasses.replace(get("hmi4k"), nl=int(name[len("hmi4k-nl"):]))
+    return get(name)
+
+
+@pytest.mark.parametrize("name", ["hmi4k", "niris64", "hinode", "hmi4k-nl12", "hmi4k-nl20"])
 def test_mlp_prediction_matches_numpy_forward(name):
     """Device inference (the fused all-layer kernel for 4*nl <= 256, the
     layer-by-layer GEMMs above) against the numpy forward pass of the same
-    exported parameters, on a pixel count that leaves a partial last tile."""
+    exported parameters, on a pixel count that leaves a partial last tile.
+    The first layer's k blocks of 32 inputs are 1 (nl 6), 2 (nl 12), 3
+    padded with zero weights to 4 (nl 20) and 8 (nl 60)."""
     import torch
-    w = get(name)
+    w = _workload(name)
     n = 3001
     ranges = w.ranges()
-    with gpu_inversion(name, precision=ME_FP32) as inv:
+    with Inversion(w.config(precision=ME_FP32), w.wavelengths()) as inv:
         inv.mlp_train(w.seed + 7, ranges, 16384, batch=1024, epochs=1, lr=2e-3, ff_frac=w.ff_frac,
                       ff_bmax=w.ff_bmax, sigma=w.sigma)
         params = inv.mlp_params()
