@@ -121,7 +121,7 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--train-epochs", type=int, default=12)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -205,6 +205,8 @@ def main():
         results["learned"] = summary(learned_ms, learned_sts)
         results["learned"]["train_s"] = round(train_s, 2)
         results["learned"]["train_loss"] = loss
+        results["learned"]["train_profiles"] = args.train_samples
+        results["learned"]["train_epochs"] = args.train_epochs
     headline = "learned" if learned_sts is not None else "plain"
     ms_per_step = learned_ms if headline == "learned" else plain_ms
     sts = learned_sts if headline == "learned" else plain_sts

@@ -13,10 +13,12 @@ LIB = CSRC / "libmeinv.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions stay off: hipcc is told the target lacks them
 # FP32 division and sqrt use the hardware reciprocal/rsq (about 1 ulp) instead
-# of the correctly rounded sequences; FP64 arithmetic is unaffected
+# of the correctly rounded sequences, and FP32 denormals flush to zero, so a
+# reciprocal is one instruction instead of a range-scaled sequence of five;
+# FP64 arithmetic is unaffected
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+         "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
          "-Wno-unused-result"]
 
 
