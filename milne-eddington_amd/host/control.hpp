@@ -1,6 +1,6 @@
 // control.hpp — MEINV control file (Src/RINPUT.c:361-563 RDINPUT, keyword
 // table :400-458, conversions of Get_Keys :48-357) -> engine configuration
-// and the driver settings (paths, solution box, chunking, caching).
+// and the run settings (paths, solution box, chunking, caching).
 #pragma once
 
 #include <string>

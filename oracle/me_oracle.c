@@ -1,8 +1,9 @@
 /*
  * me_oracle.c — CPU FP64 oracle of the Milne–Eddington inversion path.
  *
- * TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the
- * checker and by bench.py's cpu_baseline leg.  The product never links it.
+ * TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() to verify
+ * the device result and by bench.py's cpu_baseline leg.  The product never
+ * links it.
  *
  * Each routine restates the reference's arithmetic in the reference's own
  * operation order (so that the Faddeeva and SVD parts agree bit for bit with

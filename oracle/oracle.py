@@ -2,8 +2,9 @@
 harness (oracle/_ref/libref.so, present only where the reference sources were
 built).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
-checker) and bench.py's cpu_baseline leg.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (to
+verify the device result) and bench.py's cpu_baseline leg.  The product never
+imports it.
 """
 from __future__ import annotations
 
