@@ -18,12 +18,16 @@ def _random_params(w, n, seed=1):
     return p
 
 
-@pytest.mark.parametrize("name", ["niris64", "hinode", "hmi4k"])
-def test_synthesis_fp64_matches_oracle(name):
+@pytest.mark.parametrize("name,nfig", [("niris64", 6), ("hinode", 6), ("hmi4k", 6), ("niris64", 5), ("hmi4k", 5),
+                                       ("niris64", 4), ("hmi4k", 4)])
+def test_synthesis_fp64_matches_oracle(name, nfig):
+    """FP64 synthesis and Jacobian, every Faddeeva region split that
+    Nfigures selects (6: Algorithm 916 / Hui p6; 5 and 4: Humlicek W4 and Hui
+    p6, Src/FADDEEVA.c:510-549, 581-583)."""
     w = get(name)
-    orc = O.Oracle(w.config(), w.wavelengths())
+    orc = O.Oracle(w.config(nfigures=nfig), w.wavelengths())
     par = _random_params(w, 300)
-    with gpu_inversion(name) as inv:
+    with gpu_inversion(name, nfigures=nfig) as inv:
         syn, J = inv.synth(par, jac=True)
     ref, Jr = orc.synth(par, jac=True)
     scale = par[:, 7][:, None, None]
