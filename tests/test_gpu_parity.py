@@ -174,3 +174,40 @@ def test_inversion_fp64_parity_fixed_parameters(name, fixed):
           f"{ok.mean():.4f} (oracle self {floor:.4f})")
     assert ok2.mean() >= floor2 - 0.02
     assert ok.mean() >= floor - 0.03
+
+
+@pytest.mark.parametrize("name", ["niris64", "hmi4k"])
+def test_inversion_fp64_parity_regularisation(name):
+    """Get_Regul on the device (Src/LM_FIT.c:354-396): the reference's
+    default theta regularisation plus an eta term, against the oracle with the
+    per-pixel reading of H2 (no LM state before a pixel's first run, which is
+    what a pixel-parallel engine can reproduce); the oracle's agreement with
+    its own sequential reading (state carried from pixel to pixel) is printed
+    as the size of the H2 effect."""
+    over = dict(regul_theta=True, regul=(0, 1, 0, 0, 0, 0, 1, 0, 0),
+                regul_value=(0, 0.5 * np.pi, 0, 0, 0, 0, 10.0, 0, 0), regul_weight=(0, 1e-4, 0, 0, 0, 0, 2e-3, 0, 0))
+    ok, floor, out, ref = _parity(name, 512, "heuristic", **over)
+    w, orc0, prof, truth = oracle_map(name, 512)
+    seq = O.Oracle(w.config(**over), w.wavelengths()).invert(prof.astype(np.float64), h2=O.ORC_H2_SEQUENTIAL)
+    h2 = within_parity(seq["res"], ref["res"]).mean()
+    print(f"{name} regularised: {ok.mean():.4f} within tolerance (oracle self {floor:.4f}); oracle sequential vs "
+          f"per-pixel H2 {h2:.4f}")
+    assert ok.mean() >= floor - 0.03
+
+
+def test_inversion_hmi_ref_rotates_phi_only():
+    """HMI_REF (Src/LM_FIT.c:730-733): the same fit with phi moved by pi/2
+    mod pi, and the oracle's result with the same setting."""
+    w, orc0, prof, truth = oracle_map("hmi4k", 256)
+    with gpu_inversion("hmi4k") as inv:
+        base = inv.invert(prof)["res"]
+    with gpu_inversion("hmi4k", hmi_ref=1) as inv:
+        rot = inv.invert(prof)["res"]
+    keep = [k for k in range(10) if k != 2]
+    np.testing.assert_array_equal(rot[:, keep], base[:, keep])
+    phi = base[:, 2] + 0.5 * np.pi
+    phi = np.where(phi > np.pi, phi - np.pi, phi)
+    valid = base[:, 9] > 0
+    np.testing.assert_array_equal(rot[valid, 2], phi[valid])
+    ok, floor, out, ref = _parity("hmi4k", 256, "heuristic", hmi_ref=1)
+    assert ok.mean() >= floor - 0.03
