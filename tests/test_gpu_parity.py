@@ -211,3 +211,22 @@ def test_inversion_hmi_ref_rotates_phi_only():
     np.testing.assert_array_equal(rot[valid, 2], phi[valid])
     ok, floor, out, ref = _parity("hmi4k", 256, "heuristic", hmi_ref=1)
     assert ok.mean() >= floor - 0.03
+
+
+def test_region_counts_switch_changes_counters_only():
+    """me_ctx_set_region_counts: the FP32 path's per-region Faddeeva counts are
+    gathered only when on; the inversion result is the same either way, and
+    with counting on the FP32 path counts what the FP64 path counts."""
+    w, orc, prof, truth = oracle_map("hmi4k", 2048)
+    with gpu_inversion("hmi4k", precision=ME_FP32) as inv:
+        inv.set_region_counts(False)
+        off = inv.invert(prof)
+        inv.set_region_counts(True)
+        on = inv.invert(prof)
+    np.testing.assert_array_equal(on["res"], off["res"])
+    assert sum(off["stats"].n_fadd) == 0
+    n_on = np.array(on["stats"].n_fadd)
+    assert n_on.sum() > 0
+    # every synthesis evaluates 3 Faddeeva calls per wavelength
+    st = on["stats"]
+    assert n_on.sum() == 3 * w.nl * (st.n_jac + st.n_trial)
