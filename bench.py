@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--tune-pixels", type=int, default=1 << 20,
+                    help="autotune the launch shape on this many pixels first (0: keep --block/--waves-per-cu)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--train-epochs", type=int, default=12)
@@ -154,10 +156,16 @@ def main():
     log(f"bench: {w.name} n={n} nl={w.nl} {args.precision} num_run={cfg.num_run} mode={args.mode}")
 
     def timed(engine, label):
-        # timed steps run without the FP32 path's per-call region counting;
-        # one extra untimed step with it gives the event counts of the op
-        # model (the inversion is deterministic: every step does the same work)
+        # the launch shape is autotuned first (untimed) unless given; timed
+        # steps run without the FP32 path's per-call region counting; one extra
+        # untimed step with it gives the event counts of the op model (the
+        # inversion is deterministic: every step does the same work)
         engine.set_region_counts(fp64)
+        tune = None
+        if args.tune_pixels > 0 and args.block == 0 and args.waves_per_cu == 0:
+            tune = engine.autotune(d_prof.data_ptr(), min(n, args.tune_pixels))
+            log(f"{label} autotuned launch: block {tune['block_size']}, {tune['waves_per_cu']} waves per CU "
+                f"({tune['kernel_ms']:.1f} ms on {min(n, args.tune_pixels)} px)")
         for i in range(args.warmup):
             st = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
             log(f"{label} warmup {i}: {st.total_ms:.1f} ms (kernel {st.kernel_ms:.1f}), converged "
@@ -174,7 +182,7 @@ def main():
         wall = time.perf_counter() - t0
         engine.set_region_counts(True)
         counted = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
-        return wall / args.steps * 1e3, sts, counted
+        return wall / args.steps * 1e3, sts, counted, tune
 
     def summary(ms, sts):
         last = sts[-1]
@@ -189,8 +197,10 @@ def main():
     results = {}
     plain_ms = plain_sts = plain_counted = None
     if args.mode in ("plain", "both"):
-        plain_ms, plain_sts, plain_counted = timed(inv, "plain")
+        plain_ms, plain_sts, plain_counted, tune = timed(inv, "plain")
         results["plain"] = summary(plain_ms, plain_sts)
+        if tune:
+            results["plain"]["launch"] = {"block": tune["block_size"], "waves_per_cu": tune["waves_per_cu"]}
     learned_ms = learned_sts = learned_counted = None
     train_s = None
     if args.mode in ("learned", "both"):
@@ -201,8 +211,10 @@ def main():
         train_s = time.perf_counter() - t
         log(f"MLP trained on {args.train_samples} device-synthesised profiles x {args.train_epochs} epochs in "
             f"{train_s:.1f} s, final loss {loss:.4g}")
-        learned_ms, learned_sts, learned_counted = timed(linv, "learned")
+        learned_ms, learned_sts, learned_counted, tune = timed(linv, "learned")
         results["learned"] = summary(learned_ms, learned_sts)
+        if tune:
+            results["learned"]["launch"] = {"block": tune["block_size"], "waves_per_cu": tune["waves_per_cu"]}
         results["learned"]["train_s"] = round(train_s, 2)
         results["learned"]["train_loss"] = loss
         results["learned"]["train_profiles"] = args.train_samples

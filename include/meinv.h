@@ -78,8 +78,8 @@ typedef struct me_config {
     int      precision;        /* ME_FP64 | ME_FP32                         */
     int      init_mode;        /* ME_INIT_HEURISTIC | ME_INIT_LEARNED       */
     uint64_t seed;             /* Philox4x32-10 key of the restart jumps    */
-    int      block_size;       /* 0 = autotuned; else 64/128/256            */
-    int      waves_per_cu;     /* 0 = autotuned; resident waves per CU      */
+    int      block_size;       /* 0 = 256; else 64/128/256 (me_ctx_autotune) */
+    int      waves_per_cu;     /* 0 = all that fit; resident waves per CU    */
 } me_config;
 
 /* Aggregate counters and optional per-pixel arrays filled by me_invert_*.
@@ -130,6 +130,16 @@ me_status me_ctx_info(const me_ctx *ctx, int *ic, int *i0, int *i1, int *ncut, d
  * results do not depend on it.  No counterpart in the reference: the counts
  * feed the benchmark's operation model. */
 me_status me_ctx_set_region_counts(me_ctx *ctx, int on);
+
+/* Launch-shape autotuner of the persistent inversion kernel: inverts the
+ * first n_sample device profiles under every candidate block size (64, 128,
+ * 256 lanes) and resident-wave count per CU that the occupancy allows (best
+ * of two runs each, the context's init mode and precision), keeps the fastest
+ * in the context's block_size / waves_per_cu and reports it.  Results never
+ * depend on the launch shape.  No counterpart in the reference (its MPI
+ * chunk size, nprofiles, is not a kernel shape). */
+me_status me_ctx_autotune(me_ctx *ctx, const void *d_prof, int prof_is_f64, int64_t n_sample,
+                          int *block_size, int *waves_per_cu, double *kernel_ms);
 
 /*
  * Invert n pixels.  Replaces INVERSION_MULTI (Src/LM_FIT.c:746-811,

@@ -507,6 +507,59 @@ me_status me_invert_device(me_ctx *c, const void *d_prof, int prof_is_f64, int64
     return ME_OK;
 }
 
+me_status me_ctx_autotune(me_ctx *c, const void *d_prof, int prof_is_f64, int64_t n, int *block_out,
+                          int *waves_out, double *ms_out)
+{
+    if (!c || n <= 0 || !d_prof) return ME_EINVAL;
+    if (c->cfg.init_mode == ME_INIT_LEARNED && !c->mlp.trained) return ME_ENOTREADY;
+    // resident waves per CU that each block size allows (registers, LDS)
+    auto kern = c->cfg.precision == ME_FP64
+                    ? (prof_is_f64 ? (const void *)me::invert_kernel<double, double> : (const void *)me::invert_kernel<double, float>)
+                    : (prof_is_f64 ? (const void *)me::invert_kernel<float, double> : (const void *)me::invert_kernel<float, float>);
+    struct Cand { int block, waves; };
+    std::vector<Cand> cands;
+    for (int block : {64, 128, 256}) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        const int wmax = per_cu * (block / 64);
+        for (int waves = wmax; waves >= block / 64 && waves >= wmax / 2; waves -= block / 64) cands.push_back({block, waves});
+    }
+    double *d_res = nullptr;
+    HIPCHK(hipMalloc(&d_res, sizeof(double) * 10 * n));
+    const int keep_block = c->cfg.block_size, keep_waves = c->cfg.waves_per_cu;
+    int best_block = 256, best_waves = 0;
+    double best = INFINITY;
+    me_status st = ME_OK;
+    for (const Cand &k : cands) {
+        c->cfg.block_size = k.block;
+        c->cfg.waves_per_cu = k.waves;
+        double t = INFINITY;
+        for (int rep = 0; rep < 2 && st == ME_OK; rep++) {
+            me_stats s{};
+            st = me_invert_device(c, d_prof, prof_is_f64, n, d_res, nullptr, nullptr, &s);
+            if (st == ME_OK && s.kernel_ms < t) t = s.kernel_ms;
+        }
+        if (st != ME_OK) break;
+        if (t < best) {
+            best = t;
+            best_block = k.block;
+            best_waves = k.waves;
+        }
+    }
+    (void)hipFree(d_res);
+    if (st != ME_OK) {
+        c->cfg.block_size = keep_block;
+        c->cfg.waves_per_cu = keep_waves;
+        return st;
+    }
+    c->cfg.block_size = best_block;
+    c->cfg.waves_per_cu = best_waves;
+    if (block_out) *block_out = best_block;
+    if (waves_out) *waves_out = best_waves;
+    if (ms_out) *ms_out = best;
+    return ME_OK;
+}
+
 me_status me_synth_batch(me_ctx *c, const double *par, int64_t n, double *stokes, double *jac)
 {
     if (!c || n < 0 || (n > 0 && (!par || !stokes))) return ME_EINVAL;

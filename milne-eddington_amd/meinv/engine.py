@@ -92,6 +92,14 @@ class Inversion:
         (n_fadd); off saves a few per cent of kernel time."""
         _check(self.lib.me_ctx_set_region_counts(self._h, int(bool(on))), "me_ctx_set_region_counts")
 
+    def autotune(self, d_prof: int, n: int, prof_is_f64: bool = False):
+        """Pick the inversion kernel's launch shape on the first n device
+        profiles (me_ctx_autotune); the choice stays in this context."""
+        b, wv, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self.lib.me_ctx_autotune(self._h, C.c_void_p(d_prof), int(bool(prof_is_f64)), int(n), C.byref(b),
+                                        C.byref(wv), C.byref(ms)), "me_ctx_autotune")
+        return dict(block_size=b.value, waves_per_cu=wv.value, kernel_ms=ms.value)
+
     def info(self):
         ic, i0, i1, nc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         v = np.zeros(2)

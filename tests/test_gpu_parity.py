@@ -230,3 +230,31 @@ def test_region_counts_switch_changes_counters_only():
     # every synthesis evaluates 3 Faddeeva calls per wavelength
     st = on["stats"]
     assert n_on.sum() == 3 * w.nl * (st.n_jac + st.n_trial)
+
+
+def test_autotuner_picks_a_valid_shape_and_keeps_results():
+    """me_ctx_autotune: the chosen block size and resident waves are one of
+    the candidates, stay in the context, and the inversion is the same under
+    every launch shape."""
+    import torch
+    w = get("hmi4k")
+    n = 20000
+    with gpu_inversion("hmi4k", precision=ME_FP32) as inv:
+        d_prof = device_tensor((n, 4, w.nl), torch.float32)
+        inv.gen_map_device(w.seed, w.ranges(), n, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                           sigma=w.sigma)
+        d_res = device_tensor((n, 10), torch.float64)
+        inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+        before = d_res.cpu().numpy()
+        tune = inv.autotune(d_prof.data_ptr(), n)
+        print(tune)
+        assert tune["block_size"] in (64, 128, 256) and tune["waves_per_cu"] >= 1 and tune["kernel_ms"] > 0
+        assert tune["waves_per_cu"] % (tune["block_size"] // 64) == 0
+        d_res.zero_()
+        inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+        np.testing.assert_array_equal(d_res.cpu().numpy(), before)
+    for block in (64, 128):
+        with gpu_inversion("hmi4k", precision=ME_FP32, block_size=block) as inv:
+            d_res.zero_()
+            inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+            np.testing.assert_array_equal(d_res.cpu().numpy(), before)
