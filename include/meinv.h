@@ -202,6 +202,12 @@ int64_t   me_mlp_num_params(const me_ctx *ctx);
 me_status me_mlp_get_params(const me_ctx *ctx, float *params);
 me_status me_mlp_set_params(me_ctx *ctx, const float *params);
 
+/* Use a network trained elsewhere (e.g. exported from another context with
+ * me_mlp_get_params): its weights and the truth ranges [9][2] it was trained
+ * on, which decode its outputs.  The context's learned mode is ready after
+ * this call.  No counterpart in the reference. */
+me_status me_mlp_import(me_ctx *ctx, const float *params, const double *ranges);
+
 /* One forward+backward pass of the MLP on a host batch, for gradient checks:
  *   x [batch][4*nl] normalised inputs, y [batch][10] targets,
  *   grad [me_mlp_num_params] (out), returns the mean squared error. */

@@ -651,6 +651,14 @@ me_status me_mlp_set_params(me_ctx *c, const float *params)
     return me::mlp_set_params(c->mlp, c->stream, params);
 }
 
+me_status me_mlp_import(me_ctx *c, const float *params, const double *ranges)
+{
+    if (!c || !params || !ranges) return ME_EINVAL;
+    for (int k = 0; k < 18; k++)
+        if (!std::isfinite(ranges[k])) return ME_EINVAL;
+    return me::mlp_import(c->mlp, c->stream, params, ranges);
+}
+
 me_status me_mlp_grad(me_ctx *c, const float *x, const float *y, int batch, float *grad, double *loss)
 {
     if (!c || !x || !y || !grad || batch <= 0) return ME_EINVAL;

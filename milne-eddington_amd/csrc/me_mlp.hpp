@@ -480,6 +480,16 @@ struct MlpState {
     int packed = 0;
 };
 
+// output decoding ranges of the 9 parameters: [lo, hi] per parameter, as the
+// training sampler's truth ranges (an empty range is widened to width 1)
+inline void mlp_set_ranges(MlpState &m, const double *ranges)
+{
+    for (int k = 0; k < 9; k++) {
+        m.norm.lo[k] = ranges[2 * k];
+        m.norm.hi[k] = ranges[2 * k + 1] > ranges[2 * k] ? ranges[2 * k + 1] : ranges[2 * k] + 1.0;
+    }
+}
+
 // free a device buffer and forget it (a failing free at this point leaves
 // nothing to recover)
 template <typename T> inline void dev_release(T *&p)
@@ -651,10 +661,7 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
     if (st != ME_OK) return st;
     if ((st = mlp_alloc_acts(m, batch)) != ME_OK) return st;
     if ((st = mlp_tmp(m, n_train, nl)) != ME_OK) return st;
-    for (int k = 0; k < 9; k++) {
-        m.norm.lo[k] = ranges[2 * k];
-        m.norm.hi[k] = ranges[2 * k + 1] > ranges[2 * k] ? ranges[2 * k + 1] : ranges[2 * k] + 1.0;
-    }
+    mlp_set_ranges(m, ranges);
     if ((st = mlp_init_params(m, s, seed)) != ME_OK) return st;
     if ((st = gen(m.prof_tmp, m.truth_tmp, n_train)) != ME_OK) return st;
     const int64_t nb = n_train / batch;
@@ -797,6 +804,17 @@ inline me_status mlp_set_params(MlpState &m, hipStream_t s, const float *in)
     hipLaunchKernelGGL(to_bf16_kernel, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Pb, m.np);
     if (hipStreamSynchronize(s) != hipSuccess) return ME_EDEVICE;
     m.packed = 0;
+    return ME_OK;
+}
+
+// a network trained elsewhere: its exported weights and the truth ranges it
+// was trained on; the context's learned mode can use it at once
+inline me_status mlp_import(MlpState &m, hipStream_t s, const float *params, const double *ranges)
+{
+    me_status st = mlp_set_params(m, s, params);
+    if (st != ME_OK) return st;
+    mlp_set_ranges(m, ranges);
+    m.trained = 1;
     return ME_OK;
 }
 

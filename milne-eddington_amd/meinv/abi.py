@@ -101,6 +101,7 @@ ENTRY_POINTS = [
     ("me_mlp_num_params", C.c_int64, [_vp]),
     ("me_mlp_get_params", C.c_int, [_vp, _fp]),
     ("me_mlp_set_params", C.c_int, [_vp, _fp]),
+    ("me_mlp_import", C.c_int, [_vp, _fp, _dp]),
     ("me_mlp_grad", C.c_int, [_vp, _fp, _fp, C.c_int, _fp, _dp]),
     ("me_version", C.c_char_p, []),
 ]

@@ -188,6 +188,13 @@ class Inversion:
         p = np.ascontiguousarray(p, dtype=np.float32)
         _check(self.lib.me_mlp_set_params(self._h, _ptr(p, _fp)), "me_mlp_set_params")
 
+    def mlp_import(self, p, ranges):
+        """A network trained elsewhere: exported weights plus the truth ranges
+        it was trained on; learned mode is ready afterwards."""
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        rg = np.ascontiguousarray(ranges, dtype=np.float64).reshape(9, 2)
+        _check(self.lib.me_mlp_import(self._h, _ptr(p, _fp), _ptr(rg, _dp)), "me_mlp_import")
+
     def mlp_grad(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(y, dtype=np.float32)

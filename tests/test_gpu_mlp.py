@@ -159,3 +159,64 @@ def test_mlp_prediction_matches_numpy_forward(name):
     assert np.quantile(dphi, 0.999) <= 2e-2
     ds = np.abs(pred[:, 7] - ref[:, 7]) / np.maximum(np.abs(ref[:, 7]), 1e-3)
     assert np.quantile(ds, 0.999) <= 5e-3
+
+
+def test_learned_fp32_error_against_fp64():
+    """The learned form's per-parameter error: the same trained network seeds
+    the FP32 engine and the FP64 engine (pinned to the oracle by
+    test_gpu_parity) on the same map; reported as the fraction of pixels
+    within 0.1 G / 0.01 deg / 1 m/s and the median differences."""
+    import torch
+    from gpu_util import DEG, within_physical
+    w = get("hmi4k")
+    n = 20000
+    ranges = w.ranges()
+    with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED) as inv32:
+        inv32.mlp_train(w.seed + 100, ranges, 262144, batch=2048, epochs=4, lr=2e-3, ff_frac=w.ff_frac,
+                        ff_bmax=w.ff_bmax, sigma=w.sigma)
+        params = inv32.mlp_params()
+        d_prof = device_tensor((n, 4, w.nl), torch.float32)
+        inv32.gen_map_device(w.seed, ranges, n, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                             sigma=w.sigma)
+        d_res = device_tensor((n, 10), torch.float64)
+        st32 = inv32.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+        a = d_res.cpu().numpy()
+    with gpu_inversion("hmi4k", init_mode=ME_INIT_LEARNED) as inv64:
+        inv64.mlp_import(params, ranges)
+        st64 = inv64.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+        b = d_res.cpu().numpy()
+    conv_a = np.mean(a[:, 9] <= w.chisq_criteria)
+    conv_b = np.mean(b[:, 9] <= w.chisq_criteria)
+    both = (a[:, 9] <= w.chisq_criteria) & (b[:, 9] <= w.chisq_criteria)
+    phys = within_physical(a[both], b[both]).mean()
+    d = np.abs(a[both] - b[both])
+    print(f"learned FP32 vs FP64: converged {conv_a:.4f} / {conv_b:.4f}; of pixels converged in both, {phys:.4f} "
+          f"within 0.1 G / 0.01 deg / 1 m/s; median |dB| {np.median(d[:, 0]):.3g} G, |dtheta| "
+          f"{np.median(d[:, 1]) * DEG:.3g} deg, |dv| {np.median(d[:, 3]) * 1e3:.3g} m/s; fallbacks "
+          f"{st32.n_fallback} / {st64.n_fallback}")
+    assert abs(conv_a - conv_b) <= 0.02
+    assert phys >= 0.6
+
+
+def test_imported_network_predicts_like_the_trained_one():
+    """me_mlp_import: weights exported from one context plus the training
+    ranges make another context's predictions (and learned mode) identical."""
+    import torch
+    w = get("hmi4k")
+    n = 4096
+    ranges = w.ranges()
+    with gpu_inversion("hmi4k", precision=ME_FP32) as a:
+        a.mlp_train(w.seed + 3, ranges, 16384, batch=1024, epochs=1, lr=2e-3, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                    sigma=w.sigma)
+        params = a.mlp_params()
+        d_prof = device_tensor((n, 4, w.nl), torch.float32)
+        a.gen_map_device(w.seed, ranges, n, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                         sigma=w.sigma)
+        d_init = device_tensor((n, 9), torch.float64)
+        a.mlp_predict_device(d_prof.data_ptr(), n, d_init.data_ptr())
+        pa = d_init.cpu().numpy()
+    with gpu_inversion("hmi4k", precision=ME_FP32) as b:
+        b.mlp_import(params, ranges)
+        d_init.fill_(-1.0)
+        b.mlp_predict_device(d_prof.data_ptr(), n, d_init.data_ptr())
+        np.testing.assert_array_equal(d_init.cpu().numpy(), pa)
