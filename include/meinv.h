@@ -148,7 +148,8 @@ me_status me_ctx_autotune(me_ctx *ctx, const void *d_prof, int prof_is_f64, int6
  *   prof  [n][4][nl] host float32 (FITS order)
  *   res   [n][10]    host double (out)
  *   fit   [n][4][nl] host double (out, may be NULL; written when non-NULL)
- *   init  [n][9]     host double explicit starting points, or NULL
+ *   init  [n][9]     host double explicit starting points, or NULL (learned
+ *                    mode: NULL seeds each pixel with the network's prediction)
  *   stats            may be NULL
  * A bad pixel never aborts the batch: it gets a zero record and a status bit.
  */

@@ -192,7 +192,8 @@ me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double
                       const double *init, me_stats *st)
 {
     if (!c || n < 0 || (n > 0 && (!prof || !res))) return ME_EINVAL;
-    if (c->cfg.init_mode == ME_INIT_LEARNED) return ME_EINVAL;  // learned mode runs on device buffers
+    const int learned = c->cfg.init_mode == ME_INIT_LEARNED;
+    if (learned && !init && !c->mlp.trained) return ME_ENOTREADY;
     if (st) {
         int32_t *keep[4] = {st->iters, st->runs, st->trials, st->status};
         std::memset(st, 0, sizeof(*st));
@@ -204,13 +205,19 @@ me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double
     HIPCHK(c->prof.ensure(esz * 4 * nl * n));
     HIPCHK(c->res.ensure(sizeof(double) * 10 * n));
     if (fit) HIPCHK(c->fit.ensure(sizeof(double) * 4 * nl * n));
-    if (init) HIPCHK(c->init.ensure(sizeof(double) * 9 * n));
+    if (init || learned) HIPCHK(c->init.ensure(sizeof(double) * 9 * n));
     const bool want_px = st && (st->iters || st->runs || st->trials || st->status);
     if (want_px) HIPCHK(c->pxs.ensure(sizeof(int32_t) * 4 * n));
     HIPCHK(hipMemcpyAsync(c->prof.p, prof, esz * 4 * nl * n, hipMemcpyHostToDevice, c->stream));
-    if (init) HIPCHK(hipMemcpyAsync(c->init.p, init, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+    if (init) {
+        // explicit starting points; in learned mode they take the network's place
+        HIPCHK(hipMemcpyAsync(c->init.p, init, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+    } else if (learned) {
+        me_status s = me::mlp_predict(c->mlp, c->dc, c->stream, c->prof.p, esz == 8, n, (double *)c->init.p);
+        if (s != ME_OK) return s;
+    }
     me_status s = invert_dispatch(c, c->prof.p, esz == 8, n, (double *)c->res.p, fit ? (double *)c->fit.p : nullptr,
-                                  init ? (const double *)c->init.p : nullptr, 0,
+                                  (init || learned) ? (const double *)c->init.p : nullptr, learned,
                                   want_px ? (int32_t *)c->pxs.p : nullptr, st);
     if (s != ME_OK) return s;
     HIPCHK(hipMemcpyAsync(res, c->res.p, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, c->stream));

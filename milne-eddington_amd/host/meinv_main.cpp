@@ -6,6 +6,7 @@
 //
 //   meinv [control-file] [--precision fp64|fp32] [--batch N] [--max-chunks K]
 //         [--files CUBE.fits ... [--result-dir DIR]]
+//         [--learned [--mlp NET | --train-profiles N --train-epochs E] [--save-mlp NET]]
 //   meinv --dump-config [control-file]      parsed settings as JSON, no GPU
 //   meinv --fits-info FILE                  header summary as JSON, no GPU
 //   meinv --fits-read FILE FIRST COUNT      pixels as text, no GPU
@@ -19,6 +20,13 @@
 // --files inverts several cubes on the control file's wavelength grid with one
 // context (DIR/<stem>_res.fits, _fit.fits, _cache), the loop that
 // Tools/RunMEINV.py runs as one MEINV process per file.
+// --learned seeds each pixel's first LM run with the MLP initialiser (pixels
+// above chisq_criteria fall back to the plain procedure).  The network is
+// read from NET (--mlp, written by --save-mlp) or trained on the GPU before
+// the first cube on N device-synthesised profiles (default 4194304, 12
+// epochs) whose truths span the control file's bounds (theta and phi over
+// [0, pi], S over [0.5, 1.2] relative to the pixel's continuum, 30 % of the
+// pixels field-free below 50 G, noise 1e-3 S).
 #include <sys/time.h>
 
 #include <algorithm>
@@ -159,7 +167,93 @@ struct Options {
     int prec = ME_FP64;
     int64_t batch = 1 << 20;
     int64_t max_chunks = -1;
+    bool learned = false;
+    std::string mlp_in, mlp_out;
+    int64_t train_profiles = 4194304;
+    int train_epochs = 12;
 };
+
+// MLP file: magic, parameter count, the truth ranges [9][2] it was trained
+// on, then the FP32 weights in me_mlp_get_params order (little-endian)
+const char kMlpMagic[8] = {'M', 'E', 'I', 'N', 'V', 'M', 'L', 'P'};
+
+bool save_mlp(const std::string &path, const std::vector<float> &w, const double ranges[18], std::string &err)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        err = "cannot create " + path;
+        return false;
+    }
+    const int64_t np = (int64_t)w.size();
+    bool ok = std::fwrite(kMlpMagic, 1, 8, f) == 8 && std::fwrite(&np, 8, 1, f) == 1 &&
+              std::fwrite(ranges, 8, 18, f) == 18 && std::fwrite(w.data(), 4, w.size(), f) == w.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) err = path + ": write failed";
+    return ok;
+}
+
+bool load_mlp(const std::string &path, int64_t expect, std::vector<float> &w, double ranges[18], std::string &err)
+{
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    char magic[8];
+    int64_t np = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMlpMagic, 8) == 0 && std::fread(&np, 8, 1, f) == 1;
+    if (ok && np != expect) {
+        std::fclose(f);
+        err = path + ": network has " + std::to_string(np) + " parameters, this wavelength grid needs " +
+              std::to_string(expect);
+        return false;
+    }
+    if (ok) {
+        w.resize((size_t)np);
+        ok = std::fread(ranges, 8, 18, f) == 18 && std::fread(w.data(), 4, w.size(), f) == w.size();
+    }
+    std::fclose(f);
+    if (!ok) err = path + ": not a network file";
+    return ok;
+}
+
+// the network for --learned: read it, or train it on the control file's bounds
+int setup_mlp(const mehost::Control &ctl, const Options &opt, me_ctx *ctx)
+{
+    std::string err;
+    const int64_t np = me_mlp_num_params(ctx);
+    std::vector<float> w;
+    double ranges[18];
+    if (!opt.mlp_in.empty()) {
+        if (!load_mlp(opt.mlp_in, np, w, ranges, err)) return fail(err);
+        const me_status st = me_mlp_import(ctx, w.data(), ranges);
+        if (st != ME_OK) return fail("me_mlp_import failed (status " + std::to_string((int)st) + ")");
+        std::printf("meinv: network read from %s\n", opt.mlp_in.c_str());
+    } else {
+        const double pi = 3.14159265358979323846;
+        const me_config &c = ctl.cfg;
+        const double r[18] = {c.bounds[0][0], c.bounds[0][1], 0.0, pi, 0.0, pi, c.bounds[3][0], c.bounds[3][1],
+                              c.bounds[4][0], c.bounds[4][1], c.bounds[5][0], c.bounds[5][1], c.bounds[6][0],
+                              c.bounds[6][1], 0.5, 1.2, c.bounds[8][0], c.bounds[8][1]};
+        std::memcpy(ranges, r, sizeof(r));
+        double loss = 0;
+        const double t0 = now_s();
+        const me_status st = me_mlp_train(ctx, c.seed + 100, ranges, 0.3, 50.0, 1e-3, opt.train_profiles, 4096,
+                                          opt.train_epochs, 2e-3, &loss);
+        if (st != ME_OK) return fail("me_mlp_train failed (status " + std::to_string((int)st) + ")");
+        std::printf("meinv: network trained on %" PRId64 " profiles x %d epochs in %.1f s, loss %.4g\n",
+                    opt.train_profiles, opt.train_epochs, now_s() - t0, loss);
+    }
+    if (!opt.mlp_out.empty()) {
+        if (w.empty()) {
+            w.resize((size_t)np);
+            const me_status st = me_mlp_get_params(ctx, w.data());
+            if (st != ME_OK) return fail("me_mlp_get_params failed (status " + std::to_string((int)st) + ")");
+        }
+        if (!save_mlp(opt.mlp_out, w, ranges, err)) return fail(err);
+    }
+    return 0;
+}
 
 struct Paths {
     std::string data, result, fit, cache;
@@ -338,6 +432,18 @@ int main(int argc, char **argv)
             if (opt.batch < 1) return fail("--batch N needs N >= 1");
         } else if (a == "--max-chunks" && i + 1 < argc) {
             opt.max_chunks = std::atoll(argv[++i]);
+        } else if (a == "--learned") {
+            opt.learned = true;
+        } else if (a == "--mlp" && i + 1 < argc) {
+            opt.mlp_in = argv[++i];
+        } else if (a == "--save-mlp" && i + 1 < argc) {
+            opt.mlp_out = argv[++i];
+        } else if (a == "--train-profiles" && i + 1 < argc) {
+            opt.train_profiles = std::atoll(argv[++i]);
+            if (opt.train_profiles < 4096) return fail("--train-profiles N needs N >= 4096");
+        } else if (a == "--train-epochs" && i + 1 < argc) {
+            opt.train_epochs = std::atoi(argv[++i]);
+            if (opt.train_epochs < 1) return fail("--train-epochs E needs E >= 1");
         } else if (a == "--result-dir" && i + 1 < argc) {
             result_dir = argv[++i];
         } else if (a == "--files") {
@@ -365,13 +471,20 @@ int main(int argc, char **argv)
     std::vector<double> lambda((size_t)nl);
     if (!mefits::read_pixels(wav, 0, nl, lambda.data(), err)) return fail(err);
 
+    if (!opt.learned && (!opt.mlp_in.empty() || !opt.mlp_out.empty()))
+        return fail("--mlp and --save-mlp need --learned");
     me_config cfg = ctl.cfg;
     cfg.precision = opt.prec;
+    if (opt.learned) cfg.init_mode = ME_INIT_LEARNED;
     me_ctx *ctx = nullptr;
     me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
     if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");
 
-    int rc = 0;
+    int rc = opt.learned ? setup_mlp(ctl, opt, ctx) : 0;
+    if (rc) {
+        me_ctx_destroy(ctx);
+        return rc;
+    }
     if (files.empty()) {
         rc = invert_cube(ctl, opt, Paths{ctl.data_path, ctl.result_path, ctl.fit_path, ctl.cache_path}, lambda, ctx);
     } else {

@@ -126,3 +126,52 @@ def test_batch_driver_inverts_every_cube_with_one_context(tmp_path, monkeypatch)
         with Inversion(cfg, w.wavelengths()) as inv:
             ref = inv.invert(cubes[k].reshape(-1, 4, w.nl).astype(np.float64))
         np.testing.assert_array_equal(res.reshape(-1, 10), ref["res"])
+
+
+def _read_net(path):
+    raw = path.read_bytes()
+    assert raw[:8] == b"MEINVMLP"
+    npar = int(np.frombuffer(raw[8:16], dtype="<i8")[0])
+    ranges = np.frombuffer(raw[16:16 + 144], dtype="<f8").reshape(9, 2)
+    params = np.frombuffer(raw[160:160 + 4 * npar], dtype="<f4")
+    assert len(raw) == 160 + 4 * npar
+    return params, ranges
+
+
+def test_host_learned_mode_trains_saves_and_reuses_the_network(tmp_path):
+    """--learned: train on the control file's bounds and save the network,
+    then a second run reads it back; both runs and the library (same network
+    imported, learned mode) give the same records."""
+    from meinv import ME_INIT_LEARNED
+    w, _ = _setup(tmp_path, nprofiles=100)
+    out = _run(tmp_path, "--learned", "--train-profiles", 65536, "--train-epochs", 2, "--save-mlp", tmp_path / "net.bin")
+    assert "network trained on 65536 profiles" in out
+    _, res1 = _read_fits(tmp_path / "res.fits")
+    params, ranges = _read_net(tmp_path / "net.bin")
+    assert ranges[1, 1] == pytest.approx(np.pi) and ranges[7].tolist() == [0.5, 1.2]
+    (tmp_path / "res.fits").unlink()
+    out = _run(tmp_path, "--learned", "--mlp", tmp_path / "net.bin")
+    assert "network read from" in out
+    _, res2 = _read_fits(tmp_path / "res.fits")
+    np.testing.assert_array_equal(res2, res1)
+    text = (tmp_path / "input").read_text()
+    cube = _read_fits_cube(tmp_path / "cube.fits")
+    x0, x1, y0, y1 = BOX
+    box = cube[y0:y1 + 1, x0:x1 + 1].reshape(-1, 4, w.nl).astype(np.float64)
+    cfg = to_config(parse_text(text))
+    cfg.init_mode = ME_INIT_LEARNED
+    with Inversion(cfg, w.wavelengths()) as inv:
+        inv.mlp_import(params, ranges)
+        ref = inv.invert(box)
+    np.testing.assert_array_equal(res2.reshape(-1, 10), ref["res"])
+    assert np.mean(ref["res"][:, 9] <= 1.0) > 0.5
+
+
+def _read_fits_cube(path):
+    """float32 cube written by _write_fits (BITPIX -32)."""
+    raw = path.read_bytes()
+    hdr = raw[:2880].decode("ascii")
+    cards = {hdr[80 * i:80 * i + 8].strip(): hdr[80 * i + 10:80 * i + 80].split("/")[0].strip() for i in range(36)}
+    shape = [int(cards[f"NAXIS{k}"]) for k in range(int(cards["NAXIS"]), 0, -1)]
+    n = int(np.prod(shape))
+    return np.frombuffer(raw[2880:2880 + 4 * n], dtype=">f4").reshape(shape)
