@@ -202,3 +202,18 @@ def test_control_parser_errors(tmp_path, bad, msg):
     assert r.returncode != 0 and msg in r.stderr
     with pytest.raises(ControlFileError):
         _python_config(bad)
+
+
+def test_learned_mode_options_are_checked(tmp_path):
+    """--mlp / --save-mlp only with --learned; training sizes validated (all
+    before any device call)."""
+    _write_fits(tmp_path / "wav.fits", np.linspace(6172.0, 6174.0, 6), -64)
+    p = tmp_path / "input"
+    p.write_text(f"lines = 6173.34, 2.5\ndata_path = {tmp_path / 'cube.fits'}\n"
+                 f"wavelength_path = {tmp_path / 'wav.fits'}\n")
+    r = _run(p, "--mlp", tmp_path / "net.bin", check=False)
+    assert r.returncode != 0 and "need --learned" in r.stderr
+    r = _run(p, "--learned", "--train-epochs", 0, check=False)
+    assert r.returncode != 0 and "--train-epochs E needs E >= 1" in r.stderr
+    r = _run(p, "--learned", "--train-profiles", 100, check=False)
+    assert r.returncode != 0 and "--train-profiles N needs N >= 4096" in r.stderr
