@@ -686,6 +686,11 @@ template <int B, int E, typename F> __device__ __forceinline__ void sfor_down(F 
     }
 }
 
+// w and e of the QR phase, one column per lane of the block (blocks of at
+// most 256 lanes)
+__shared__ double g_svd_w[9][256];
+__shared__ double g_svd_e[9][256];
+
 // returns 0, 1 when a singular value needed 50 QR sweeps, 2 for a non-finite matrix
 __device__ __forceinline__ int svd9(double a[81], double w[9])
 {
@@ -808,7 +813,16 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     //    that is not done.  A lane whose k has converged moves on to k - 1 at
     //    once instead of waiting for the slowest lane of the wave, so the wave
     //    runs max(sum of steps) rather than sum over k of max(steps at k).
-    //    Indices that depend on k or L are only used through selects.
+    //    w and e live in the lane's LDS columns during this phase, so entries
+    //    at k, k - 1 and L are addressed instead of picked by nine-way selects.
+    double *W = &g_svd_w[0][threadIdx.x], *E = &g_svd_e[0][threadIdx.x];
+#define QW(i) W[(i) * 256]
+#define QE(i) E[(i) * 256]
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        QW(i) = w[i];
+        QE(i) = e[i];
+    }
     int fail = 0;
     int k = 8, its = 0;
 #pragma unroll 1
@@ -817,8 +831,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         unsigned nme = 0, nmw = 0;
 #pragma unroll
         for (int ll = 0; ll < 9; ll++) {
-            nme |= (fabs(e[ll]) + anorm == anorm) ? (1u << ll) : 0u;
-            nmw |= (fabs(w[ll]) + anorm == anorm) ? (2u << ll) : 0u;
+            nme |= (fabs(QE(ll)) + anorm == anorm) ? (1u << ll) : 0u;
+            nmw |= (fabs(QW(ll)) + anorm == anorm) ? (2u << ll) : 0u;
         }
         // split search from l = k down (e[l] tested before w[l-1]); lanes
         // whose k converged without a cancellation take the next k
@@ -832,9 +846,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             const bool conv = k >= 0 && L == k && !cancel;
             if (__ballot(conv) == 0) break;
             if (conv) {
-#pragma unroll
-                for (int kk = 0; kk < 9; kk++)
-                    if (kk == k && w[kk] < 0.0) w[kk] = -w[kk];
+                const double wk = QW(k);
+                if (wk < 0.0) QW(k) = -wk;
                 k--;
                 its = 0;
             }
@@ -856,15 +869,16 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
 #pragma unroll
             for (int i = 1; i < 9; i++) {
                 if (!stop && i >= L && i <= k) {
-                    double f = s * e[i];
-                    e[i] = c * e[i];
+                    const double ei = QE(i);
+                    double f = s * ei;
+                    QE(i) = c * ei;
                     if (fabs(f) + anorm == anorm) {
                         stop = true;
                     } else {
-                        double gg = w[i];
+                        double gg = QW(i);
                         double h, rh;
                         hyp_r(f, gg, h, rh);
-                        w[i] = h;
+                        QW(i) = h;
                         c = gg * rh;
                         s = -f * rh;
 #pragma unroll
@@ -884,20 +898,12 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 }
         }
         if (k >= 0) {
-            double z = 0.0, wkm1 = 0.0, ek = 0.0, ekm1 = 0.0, x = 0.0;
-#pragma unroll
-            for (int kk = 0; kk < 9; kk++) {
-                if (kk == k) { z = w[kk]; ek = e[kk]; }
-                if (kk == k - 1) { wkm1 = w[kk]; ekm1 = e[kk]; }
-                if (kk == L) x = w[kk];
-            }
+            double z = QW(k);
+            const double ek = QE(k), wkm1 = k > 0 ? QW(k - 1) : 0.0, ekm1 = k > 0 ? QE(k - 1) : 0.0;
+            double x = QW(L);
             if (L == k) {
                 // converged right after a cancellation
-                if (z < 0.0) {
-#pragma unroll
-                    for (int kk = 0; kk < 9; kk++)
-                        if (kk == k) w[kk] = -z;
-                }
+                if (z < 0.0) QW(k) = -z;
                 k--;
                 its = 0;
             } else {
@@ -913,13 +919,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 for (int j = 0; j < 8; j++) {
                     if (j < k && j >= L) {
                         const int i = j + 1;
-                        gg = e[i];
-                        yv = w[i];
+                        gg = QE(i);
+                        yv = QW(i);
                         h = s * gg;
                         gg *= c;
                         double rz;
                         hyp_r(f, h, z, rz);
-                        e[j] = z;
+                        QE(j) = z;
                         c = f * rz;
                         s = h * rz;
                         f = x * c + gg * s;
@@ -927,7 +933,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         h = yv * s;
                         yv *= c;
                         hyp_r(f, h, z, rz);
-                        w[j] = z;
+                        QW(j) = z;
                         if (z != 0.0) {
                             c = f * rz;
                             s = h * rz;
@@ -942,14 +948,9 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         }
                     }
                 }
-#pragma unroll
-                for (int kk = 0; kk < 9; kk++) {
-                    if (kk == L) e[kk] = 0.0;
-                }
-#pragma unroll
-                for (int kk = 0; kk < 9; kk++) {
-                    if (kk == k) { e[kk] = f; w[kk] = x; }
-                }
+                QE(L) = 0.0;
+                QE(k) = f;
+                QW(k) = x;
                 // 50 sweeps without convergence: flagged, go on with k - 1
                 if (its >= 50) {
                     k--;
@@ -958,6 +959,10 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             }
         }
     }
+#pragma unroll
+    for (int i = 0; i < 9; i++) w[i] = QW(i);
+#undef QW
+#undef QE
     return fail;
 }
 
