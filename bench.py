@@ -123,7 +123,7 @@ def main():
                     help="autotune the launch shape on this many pixels first (0: keep --block/--waves-per-cu)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
-    ap.add_argument("--train-epochs", type=int, default=12)
+    ap.add_argument("--train-epochs", type=int, default=24)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()

@@ -170,7 +170,7 @@ struct Options {
     bool learned = false;
     std::string mlp_in, mlp_out;
     int64_t train_profiles = 4194304;
-    int train_epochs = 12;
+    int train_epochs = 24;
 };
 
 // MLP file: magic, parameter count, the truth ranges [9][2] it was trained
