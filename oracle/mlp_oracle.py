@@ -1,5 +1,5 @@
 """CPU FP64 oracle of the learned initialiser's MLP (4*nl -> 256 -> 256 ->
-256 -> 10, GELU), forward and backward.
+256 -> 10, tanh-form GELU), forward and backward.
 
 TEST INFRASTRUCTURE ONLY (imported by tests/): checks the device BF16-MFMA
 training kernels.  With ``bf16=True`` the operands are rounded to bfloat16 at
@@ -25,12 +25,27 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+K_GELU = 2.0 * np.sqrt(2.0 / np.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU in its tanh form, z * sigmoid(k (z + 0.044715 z^3)), k = 2 sqrt(2/pi):
+    the activation the device network uses (within 5e-4 of z Phi(z))."""
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(over="ignore"):
+        return z / (1.0 + np.exp(-K_GELU * (z + 0.044715 * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(over="ignore"):
+        s = 1.0 / (1.0 + np.exp(-K_GELU * (z + 0.044715 * z ** 3)))
+    return s + z * s * (1.0 - s) * K_GELU * (1.0 + 3 * 0.044715 * z * z)
+
+
+def gelu_erf(z):
+    """z Phi(z), the exact GELU the tanh form approximates."""
+    return 0.5 * z * (1.0 + erf(np.asarray(z, dtype=np.float64) / np.sqrt(2.0)))
 
 
 def unpack(params, nin):

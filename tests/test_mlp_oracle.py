@@ -34,3 +34,20 @@ def test_backward_against_finite_differences():
         pm[i] -= h
         fd = (M.loss_and_grad(pp, x, y, bf=False)[0] - M.loss_and_grad(pm, x, y, bf=False)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max()), i
+
+
+def test_gelu_tanh_form_against_exact_gelu_and_its_derivative():
+    z = np.linspace(-12.0, 12.0, 24001)
+    # the tanh form stays within 5e-4 of z Phi(z) (largest gap near |z| ~ 2)
+    assert np.max(np.abs(M.gelu(z) - M.gelu_erf(z))) < 5e-4
+    # known values of the tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+    for v in (-3.0, -1.0, -0.25, 0.5, 1.0, 2.5):
+        ref = 0.5 * v * (1.0 + np.tanh(np.sqrt(2.0 / np.pi) * (v + 0.044715 * v ** 3)))
+        assert abs(M.gelu(v) - ref) < 1e-14
+    assert abs(M.gelu(1.0) - 0.8411919906082768) < 1e-12
+    # saturation: z for large positive z, 0 for large negative z
+    assert M.gelu(60.0) == 60.0 and abs(M.gelu(-60.0)) == 0.0
+    # derivative against central differences
+    h = 1e-5
+    fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
+    assert np.max(np.abs(M.gelu_grad(z) - fd)) < 1e-8
