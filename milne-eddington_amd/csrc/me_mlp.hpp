@@ -283,8 +283,13 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__
 // consecutive output channels per 16-channel tile; the next layer reads its
 // k index in that order, so the packed weights of layers 1..3 permute their
 // input rows to match (mlp_in_channel) and no data moves between lanes.
-constexpr int MI_C = 4;                  // pixel tiles (16 each) per wave
-constexpr int MI_PX = 4 * 16 * MI_C;     // pixels per block pass
+// Two tiles per wave and eight waves per block (two per SIMD: 256 VGPRs
+// each, no spills) run faster than four tiles and four waves (one per SIMD),
+// whose MFMA, LDS and GELU phases do not overlap.
+constexpr int MI_C = 2;                  // pixel tiles (16 each) per wave
+constexpr int MI_W = 8;                  // waves per block
+constexpr int MI_T = 64 * MI_W;          // threads per block
+constexpr int MI_PX = MI_W * 16 * MI_C;  // pixels per block pass
 
 // input channel that k element q of k-block kb, lane group g reads
 __device__ __forceinline__ int mlp_in_channel(int layer, int kb, int g, int q)
@@ -367,7 +372,7 @@ inline int mlp_kb0_padded(int ninp)
 }
 
 template <typename TIn>
-__global__ __launch_bounds__(256) void mlp_infer_kernel(const TIn *__restrict__ prof, int64_t n, int nl, int kb0,
+__global__ __launch_bounds__(MI_T) void mlp_infer_kernel(const TIn *__restrict__ prof, int64_t n, int nl, int kb0,
                                                         const bf16x8 *__restrict__ wpk, MlpPack d,
                                                         const float *__restrict__ bpk, MlpNorm nm,
                                                         double *__restrict__ init)
@@ -409,8 +414,8 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const TIn *__restrict__ 
             const int KB = d.KB[l], T = d.T[l];
             __syncthreads();
             const bf16x8 *src = wpk + d.frag[l] * 64;
-            for (int i = tid; i < T * KB * 64; i += 256) sW[i] = src[i];
-            sb[tid] = bpk[l * 256 + tid];
+            for (int i = tid; i < T * KB * 64; i += MI_T) sW[i] = src[i];
+            for (int i = tid; i < 256; i += MI_T) sb[i] = bpk[l * 256 + i];
             __syncthreads();
             if (l < MLP_NL - 1) {
                 if (KB == 8) mlp_hidden_layer<8>(sW, sb, lane, g, act);
@@ -740,10 +745,10 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
         const unsigned grid = (unsigned)(passes < ncu ? passes : ncu);
         if (grid == 0) return ME_OK;
         if (prof_is_f64)
-            hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(grid), dim3(256), 0, s, (const double *)prof, n, nl,
+            hipLaunchKernelGGL(mlp_infer_kernel<double>, dim3(grid), dim3(MI_T), 0, s, (const double *)prof, n, nl,
                                m.ninp / 32, (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
         else
-            hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(grid), dim3(256), 0, s, (const float *)prof, n, nl,
+            hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(grid), dim3(MI_T), 0, s, (const float *)prof, n, nl,
                                m.ninp / 32, (const bf16x8 *)m.wpk, m.pack, (const float *)m.bpk, m.norm, init);
         return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
     }
