@@ -649,20 +649,21 @@ __device__ __forceinline__ double rsq_nr(double q)  // q normal, finite, > 0
     e = fma(-t, r, 1.0);
     return fma(0.5 * r, e, r);
 }
-// z = hypot(f, h) and rz = 1 / z (0 when z = 0); squares stay in range on
-// the fast path, the scaled form covers the rest
+// z = hypot(f, h) and rz = 1 / z (both 0 when f = h = 0), without branches:
+// f and h are scaled by 2^-e, e the exponent of max(|f|, |h|), so the sum of
+// squares lies in [0.25, 2) whatever their magnitude; the scaling and the
+// unscaling are exact.  A branch between an unscaled form and a guarded one
+// costs more in the QR sweep than the four ldexp.
 __device__ __forceinline__ void hyp_r(double f, double h, double &z, double &rz)
 {
     const double af = fabs(f), ah = fabs(h);
     const double mx = af > ah ? af : ah;
-    if (mx > 1e-140 && mx < 1e140) {
-        const double q = f * f + h * h;
-        rz = rsq_nr(q);
-        z = q * rz;
-    } else {
-        z = hyp(f, h);
-        rz = z != 0.0 ? 1.0 / z : 0.0;
-    }
+    const int ex = __builtin_amdgcn_frexp_exp(mx);   // mx = m 2^ex, m in [0.5, 1)
+    const double fs = __builtin_amdgcn_ldexp(f, -ex), hs = __builtin_amdgcn_ldexp(h, -ex);
+    const double q = fs * fs + hs * hs;
+    const double r = rsq_nr(q);
+    z = mx == 0.0 ? 0.0 : __builtin_amdgcn_ldexp(q * r, ex);
+    rz = mx == 0.0 ? 0.0 : __builtin_amdgcn_ldexp(r, -ex);
 }
 
 // keeps the scheduler from hoisting work across SVD steps (live ranges of
