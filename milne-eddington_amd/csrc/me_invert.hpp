@@ -263,6 +263,128 @@ __device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const vo
     return PassRes{chi / (4. * nl), fc.lo, fc.hi};
 }
 
+// position of the n-th set bit (n from 0) of m, which has more than n set bits
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = (1ull << w) - 1ull;
+        const int c = __popcll(m & low);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m &= low;
+        }
+    }
+    return pos;
+}
+
+// The Jacobian pass of the FP32 path spread over the whole wave.  The
+// wavelengths of a pixel are cut into two fixed halves [0, nl/2) and
+// [nl/2, nl); its normal matrix, gradient and chi2 are the sum of the two
+// halves' partial sums (each in wavelength order), whichever lanes compute
+// them, so results do not depend on how pixels fall into waves.  The lanes
+// that need a normal matrix (`own`) are ranked r = 0 .. nJ-1; work item
+// i = h nJ + r (half h of pixel r) goes to lane i mod 64 in round i div 64:
+// one round of nl/2 wavelengths while nJ <= 32, instead of nl per lane.
+// Owners add the halves in order and write them like jac_pass.  Returns the
+// owner's chi2 and this lane's own Faddeeva branch counts.  Called by the
+// whole wave.
+template <typename T, typename TIn>
+__device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, bool own,
+                                          int64_t base, P9 par, double step7, double ivn, double *__restrict__ mat,
+                                          double *__restrict__ gout, int64_t NS)
+{
+    const DevCfg &C = *Cp;
+    const int nl = C.nl;
+    const int nh = nl >= 2 ? 2 : 1;                           // halves per pixel
+    const uint64_t mask = __ballot(own);
+    const int nJ = __popcll(mask);
+    const int lane = __lane_id();
+    const int rk = __popcll(mask & lanemask_lt());            // owner's rank
+    const int rounds = (nh * nJ + 63) / 64;
+    T H[45], G[9];
+#pragma unroll
+    for (int q = 0; q < 45; q++) H[q] = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) G[q] = 0;
+    double chs = 0.0;
+    FCount fc;
+    for (int t = 0; t < rounds; t++) {
+        const int i = 64 * t + lane;
+        const int h = i / nJ, r = i % nJ;
+        const bool work = i < nh * nJ;
+        const int src = nth_set_bit(mask, r);                 // owner of pixel r
+        P9 p;
+#pragma unroll
+        for (int q = 0; q < 9; q++) p.v[q] = __shfl(par.v[q], src);
+        const double s7 = __shfl(step7, src), iv = __shfl(ivn, src);
+        const int64_t b = __shfl(base, src);
+        const int l0 = work ? (h == 0 ? 0 : nl / 2) : 0;
+        const int l1 = work ? (h == 0 && nh == 2 ? nl / 2 : nl) : 0;
+        PixK<T> K;
+        pix_consts<T>(C, p.v, K);
+        const T ivT[4] = {T(C.wsq[0] * iv), T(C.wsq[1] * iv), T(C.wsq[2] * iv), T(C.wsq[3] * iv)};
+        T hacc[45], gacc[9];
+#pragma unroll
+        for (int q = 0; q < 45; q++) hacc[q] = 0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) gacc[q] = 0;
+        double chi = 0.0;
+        for (int iw = l0; iw < l1; iw++) {
+            T o[4], d[4];
+            HessSink<T> sink{hacc, gacc, d, ivT};
+            me_point<T, true>(C, K, T(s7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+                if (s == 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        d[q] = o[q] - T(ld<TIn>(prof, b + q * nl + iw));
+                        chi += (double)d[q] * (double)d[q] * (C.wsq[q] * iv);
+                    }
+                }
+                sink(s, row);
+            }, &fc);
+        }
+        // the owner of rank rk takes half 0 (item rk, always in round 0) and
+        // then half 1 (item nJ + rk) when they fall into this round
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int it = hh * nJ + rk;
+            const bool take = own && hh < nh && it / 64 == t;
+            const int from = it % 64;
+#pragma unroll
+            for (int q = 0; q < 45; q++) {
+                const T v = __shfl(hacc[q], from);
+                if (take) H[q] = hh == 0 ? v : H[q] + v;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const T v = __shfl(gacc[q], from);
+                if (take) G[q] = hh == 0 ? v : G[q] + v;
+            }
+            const double v = __shfl(chi, from);
+            if (take) chs = hh == 0 ? v : chs + v;
+        }
+    }
+    if (own) {
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < 9; a++) {
+#pragma unroll
+            for (int bb = a; bb < 9; bb++) {
+                mat[(int64_t)(a * 9 + bb) * NS] = (double)H[idx];
+                if (bb != a) mat[(int64_t)(bb * 9 + a) * NS] = (double)H[idx];
+                idx++;
+            }
+            gout[(int64_t)a * NS] = (double)G[a];
+        }
+    }
+    return PassRes{chs / (4. * nl), fc.lo, fc.hi};
+}
+
 // Trial synthesis at pt: chi2 only (Loss_Function of Src/LM_FIT.c:89-120)
 template <typename T, typename TIn>
 __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
@@ -499,17 +621,40 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
         const int64_t base = pix * 4 * nl;
 
         // ---------------------------------------------- Jacobian + normal matrix
-        if (phase == PH_JAC_START || phase == PH_JAC) {
-            P9 pv;
+        // FP32 path with nl >= 12: the lanes that need a normal matrix share
+        // the pass with the rest of the wave (jac_split; at nl = 6 the halved
+        // wavelength loop no longer pays for the exchange).  Otherwise, and on
+        // the FP64 path, each lane runs its own pixel in the reference's
+        // summation order.
+        const bool jl = phase == PH_JAC_START || phase == PH_JAC;
+        const bool split = sizeof(T) == 4 && nl >= 12;
+        PassRes jr{0.0, 0, 0};
+        if (split) {
+            if (__ballot(jl) != 0) {
+                P9 pv;
 #pragma unroll
-            for (int k = 0; k < 9; k++) pv.v[k] = par[k];
+                for (int k = 0; k < 9; k++) pv.v[k] = par[k];
+                jr = jac_split<T, TIn>(Cp, A.prof, jl, base, pv, step7, ivn, scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
+                                       scr + (int64_t)SL_G * NS, NS);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
+            }
+        }
+        if (jl) {
             double *mat = scr + (int64_t)(SL_MAT + 81 * mbuf) * NS;
-            const PassRes jr = jac_pass<T, TIn>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
+            if (!split) {
+                P9 pv;
+#pragma unroll
+                for (int k = 0; k < 9; k++) pv.v[k] = par[k];
+                jr = jac_pass<T, TIn>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
+            }
             const double chi = jr.chi;
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
             CT(CT_JAC) += 1;
             CT(CT_HESS) += 1;
             if (phase == PH_JAC_START) {
