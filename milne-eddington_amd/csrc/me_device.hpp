@@ -649,21 +649,24 @@ __device__ __forceinline__ double rsq_nr(double q)  // q normal, finite, > 0
     e = fma(-t, r, 1.0);
     return fma(0.5 * r, e, r);
 }
-// z = hypot(f, h) and rz = 1 / z (both 0 when f = h = 0), without branches:
-// f and h are scaled by 2^-e, e the exponent of max(|f|, |h|), so the sum of
-// squares lies in [0.25, 2) whatever their magnitude; the scaling and the
-// unscaling are exact.  A branch between an unscaled form and a guarded one
-// costs more in the QR sweep than the four ldexp.
-__device__ __forceinline__ void hyp_r(double f, double h, double &z, double &rz)
+// z = hypot(f, h) with cf = f / z and ch = h / z (the Givens pair), without
+// branches: f and h are scaled by 2^-e, e the exponent of max(|f|, |h|), so
+// the sum of squares q lies in [0.25, 2) whatever their magnitude; the
+// scaling and the unscaling are exact, and f / z = (f 2^-e) / sqrt(q) needs no
+// unscaling at all.  For f = h = 0, q = 0, z = 0 and cf = ch = 0 (the
+// reciprocal square root is taken of max(q, 0.25), which leaves every other
+// case unchanged).  A branch between an unscaled form and a guarded one costs
+// more in the QR sweep than the ldexp.
+__device__ __forceinline__ void hyp_cs(double f, double h, double &z, double &cf, double &ch)
 {
-    const double af = fabs(f), ah = fabs(h);
-    const double mx = af > ah ? af : ah;
+    const double mx = fmax(fabs(f), fabs(h));
     const int ex = __builtin_amdgcn_frexp_exp(mx);   // mx = m 2^ex, m in [0.5, 1)
     const double fs = __builtin_amdgcn_ldexp(f, -ex), hs = __builtin_amdgcn_ldexp(h, -ex);
     const double q = fs * fs + hs * hs;
-    const double r = rsq_nr(q);
-    z = mx == 0.0 ? 0.0 : __builtin_amdgcn_ldexp(q * r, ex);
-    rz = mx == 0.0 ? 0.0 : __builtin_amdgcn_ldexp(r, -ex);
+    const double r = rsq_nr(fmax(q, 0.25));
+    z = __builtin_amdgcn_ldexp(q * r, ex);
+    cf = fs * r;
+    ch = hs * r;
 }
 
 // keeps the scheduler from hoisting work across SVD steps (live ranges of
@@ -877,11 +880,11 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         stop = true;
                     } else {
                         double gg = QW(i);
-                        double h, rh;
-                        hyp_r(f, gg, h, rh);
+                        double h, cf, cg;
+                        hyp_cs(f, gg, h, cf, cg);
                         QW(i) = h;
-                        c = gg * rh;
-                        s = -f * rh;
+                        c = cg;
+                        s = -cf;
 #pragma unroll
                         for (int r = 0; r < 9; r++) {
                             double yv = col[r], zv = a[r * 9 + i];
@@ -924,20 +927,18 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         yv = QW(i);
                         h = s * gg;
                         gg *= c;
-                        double rz;
-                        hyp_r(f, h, z, rz);
+                        hyp_cs(f, h, z, c, s);
                         QE(j) = z;
-                        c = f * rz;
-                        s = h * rz;
                         f = x * c + gg * s;
                         gg = gg * c - x * s;
                         h = yv * s;
                         yv *= c;
-                        hyp_r(f, h, z, rz);
+                        double c2, s2;
+                        hyp_cs(f, h, z, c2, s2);
                         QW(j) = z;
                         if (z != 0.0) {
-                            c = f * rz;
-                            s = h * rz;
+                            c = c2;
+                            s = s2;
                         }
                         f = c * gg + s * yv;
                         x = c * yv - s * gg;
