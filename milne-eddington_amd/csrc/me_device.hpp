@@ -703,12 +703,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     bool finite = true;
 #pragma unroll
     for (int i = 0; i < 81; i++) finite = finite && isfinite(a[i]);
+    // a non-finite matrix is decomposed as the zero matrix (which converges at
+    // once) and returned as U = 0, w = 0 with status 2.  No early return: an
+    // exit in the middle makes the compiler carry every entry of a and w through
+    // a merge of two paths (over a thousand register copies per call).
     if (!finite) {
 #pragma unroll
         for (int i = 0; i < 81; i++) a[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < 9; i++) w[i] = 0.0;
-        return 2;
     }
     // 1. bidiagonalisation
     sfor_up<0, 9>([&](auto I) __attribute__((always_inline)) {
@@ -965,6 +966,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     for (int i = 0; i < 9; i++) w[i] = QW(i);
 #undef QW
 #undef QE
+    if (!finite) {
+#pragma unroll
+        for (int i = 0; i < 81; i++) a[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) w[i] = 0.0;
+        fail = 2;
+    }
     return fail;
 }
 
