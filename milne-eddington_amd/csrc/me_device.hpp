@@ -616,16 +616,6 @@ __device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T st
 // EISPACK/Numerical-Recipes formulation the reference uses, so U has the same
 // column signs and order.  V is not accumulated: U and w never depend on it,
 // and the solve rebuilds V t = A^T U W^-1 t from the input matrix A.
-__device__ __forceinline__ double hyp(double a, double b)
-{
-    double aa = a > 0 ? a : -a, ab = b > 0 ? b : -b;
-    double mx = aa > ab ? aa : ab;
-    if (mx == 0) return 0;
-    if (mx < 1e150) return sqrt(a * a + b * b);
-    double mn = aa > ab ? ab : aa;
-    double r = mn / mx;
-    return mx * sqrt(1.0 + r * r);
-}
 
 // Reciprocal and reciprocal square root from the hardware estimates with two
 // Newton steps (each doubles the correct bits: ~1 ulp at the end).  The SVD
@@ -792,7 +782,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
 #pragma unroll
         for (int j = l; j < 9; j++) a[i * 9 + j] = 0.0;
         if (gg != 0.0) {
-            gg = 1.0 / gg;
+            gg = rcp_nr(gg);
             const double rii = rcp_nr(a[i * 9 + i]);
             sfor_up<l, 9>([&](auto Jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(Jc)::value;
@@ -915,10 +905,14 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 if (its == 49) fail = 1;
                 its++;
                 double yv = wkm1, gg = ekm1, h = ek;
-                double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) / (2.0 * h * yv);
-                gg = hyp(f, 1.0);
+                // Wilkinson shift from the trailing 2x2 block; its divisions
+                // and hypot(f, 1) use the Newton-refined reciprocal and the
+                // branch-free hypot (the shift only steers convergence)
+                double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) * rcp_nr(2.0 * h * yv);
+                double cf_, c1_;
+                hyp_cs(f, 1.0, gg, cf_, c1_);
                 double sg = f >= 0.0 ? gg : -gg;
-                f = ((x - z) * (x + z) + h * (yv / (f + sg) - h)) / x;
+                f = ((x - z) * (x + z) + h * (yv * rcp_nr(f + sg) - h)) * rcp_nr(x);
                 double c = 1.0, s = 1.0;
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
