@@ -719,7 +719,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 s += a[k * 9 + i] * a[k * 9 + i];
             }
             double f = a[i * 9 + i];
-            g = f >= 0 ? -sqrt(s) : sqrt(s);
+            const double rts = s * rsq_nr(s);   // s in [1/9, 1]: entries scaled by their 1-norm
+            g = f >= 0 ? -rts : rts;
             double h = f * g - s;
             a[i * 9 + i] = f - g;
             const double rh = rcp_nr(h);
@@ -751,7 +752,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                     s += a[i * 9 + k] * a[i * 9 + k];
                 }
                 double f = a[i * 9 + l];
-                g = f >= 0 ? -sqrt(s) : sqrt(s);
+                const double rts = s * rsq_nr(s);
+                g = f >= 0 ? -rts : rts;
                 double h = f * g - s;
                 a[i * 9 + l] = f - g;
                 double rv[9];
