@@ -826,10 +826,22 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     for (int guard = 0; guard < 9 * 50 + 64; guard++) {
         // negligible e[l] (bit l of nme) and w[l-1] (bit l of nmw); e[0] is 0
         unsigned nme = 0, nmw = 0;
+        {
+            // all eighteen LDS reads first, then the tests: one LDS round trip
+            // per sweep instead of one per pair
+            double ev[9], wv[9];
 #pragma unroll
-        for (int ll = 0; ll < 9; ll++) {
-            nme |= (fabs(QE(ll)) + anorm == anorm) ? (1u << ll) : 0u;
-            nmw |= (fabs(QW(ll)) + anorm == anorm) ? (2u << ll) : 0u;
+            for (int ll = 0; ll < 9; ll++) {
+                ev[ll] = QE(ll);
+                wv[ll] = QW(ll);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x0100, 18, 0);
+            __builtin_amdgcn_sched_group_barrier(0x0002, 200, 0);
+#pragma unroll
+            for (int ll = 0; ll < 9; ll++) {
+                nme |= (fabs(ev[ll]) + anorm == anorm) ? (1u << ll) : 0u;
+                nmw |= (fabs(wv[ll]) + anorm == anorm) ? (2u << ll) : 0u;
+            }
         }
         // split search from l = k down (e[l] tested before w[l-1]); lanes
         // whose k converged without a cancellation take the next k
