@@ -826,22 +826,22 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     for (int guard = 0; guard < 9 * 50 + 64; guard++) {
         // negligible e[l] (bit l of nme) and w[l-1] (bit l of nmw); e[0] is 0
         unsigned nme = 0, nmw = 0;
-        {
-            // all eighteen LDS reads first, then the tests: one LDS round trip
-            // per sweep instead of one per pair
-            double ev[9], wv[9];
+        // all eighteen LDS reads first, then the tests: one LDS round trip per
+        // sweep instead of one per pair.  ev / wv then mirror e and w for the
+        // rest of the sweep at every index the cancellation and the rotation
+        // steps read (those below the lane's k, which only they modify)
+        double ev[9], wv[9];
 #pragma unroll
-            for (int ll = 0; ll < 9; ll++) {
-                ev[ll] = QE(ll);
-                wv[ll] = QW(ll);
-            }
-            __builtin_amdgcn_sched_group_barrier(0x0100, 18, 0);
-            __builtin_amdgcn_sched_group_barrier(0x0002, 200, 0);
+        for (int ll = 0; ll < 9; ll++) {
+            ev[ll] = QE(ll);
+            wv[ll] = QW(ll);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x0100, 18, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0002, 200, 0);
 #pragma unroll
-            for (int ll = 0; ll < 9; ll++) {
-                nme |= (fabs(ev[ll]) + anorm == anorm) ? (1u << ll) : 0u;
-                nmw |= (fabs(wv[ll]) + anorm == anorm) ? (2u << ll) : 0u;
-            }
+        for (int ll = 0; ll < 9; ll++) {
+            nme |= (fabs(ev[ll]) + anorm == anorm) ? (1u << ll) : 0u;
+            nmw |= (fabs(wv[ll]) + anorm == anorm) ? (2u << ll) : 0u;
         }
         // split search from l = k down (e[l] tested before w[l-1]); lanes
         // whose k converged without a cancellation take the next k
@@ -878,16 +878,18 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
 #pragma unroll
             for (int i = 1; i < 9; i++) {
                 if (!stop && i >= L && i <= k) {
-                    const double ei = QE(i);
+                    const double ei = ev[i];
                     double f = s * ei;
-                    QE(i) = c * ei;
+                    ev[i] = c * ei;
+                    QE(i) = ev[i];
                     if (fabs(f) + anorm == anorm) {
                         stop = true;
                     } else {
-                        double gg = QW(i);
+                        double gg = wv[i];
                         double h, cf, cg;
                         hyp_cs(f, gg, h, cf, cg);
                         QW(i) = h;
+                        wv[i] = h;
                         c = cg;
                         s = -cf;
 #pragma unroll
@@ -932,8 +934,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 for (int j = 0; j < 8; j++) {
                     if (j < k && j >= L) {
                         const int i = j + 1;
-                        gg = QE(i);
-                        yv = QW(i);
+                        gg = ev[i];
+                        yv = wv[i];
                         h = s * gg;
                         gg *= c;
                         hyp_cs(f, h, z, c, s);
