@@ -822,6 +822,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     }
     int fail = 0;
     int k = 8, its = 0;
+    unsigned convm = 0;   // bit i: singular value i converged
 #pragma unroll 1
     for (int guard = 0; guard < 9 * 50 + 64; guard++) {
         // negligible e[l] (bit l of nme) and w[l-1] (bit l of nmw); e[0] is 0
@@ -855,8 +856,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             const bool conv = k >= 0 && L == k && !cancel;
             if (__ballot(conv) == 0) break;
             if (conv) {
-                const double wk = QW(k);
-                if (wk < 0.0) QW(k) = -wk;
+                convm |= 1u << k;   // its sign is made non-negative at the end
                 k--;
                 its = 0;
             }
@@ -914,7 +914,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             double x = QW(L);
             if (L == k) {
                 // converged right after a cancellation
-                if (z < 0.0) QW(k) = -z;
+                convm |= 1u << k;
                 k--;
                 its = 0;
             } else {
@@ -972,8 +972,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             }
         }
     }
+    // converged singular values are made non-negative (the sign change of the
+    // reference's convergence step, deferred: no later step reads them)
 #pragma unroll
-    for (int i = 0; i < 9; i++) w[i] = QW(i);
+    for (int i = 0; i < 9; i++) {
+        const double wi = QW(i);
+        w[i] = ((convm >> i) & 1u) != 0u && wi < 0.0 ? -wi : wi;
+    }
 #undef QW
 #undef QE
     if (!finite) {
