@@ -235,14 +235,25 @@ __device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const vo
     for (int k = 0; k < 9; k++) gacc[k] = 0;
     double chi = 0.0;
     FCount fc;
+    // the observed Stokes values of the next wavelength are loaded while this
+    // one is synthesised, so the loop does not wait on memory each step
+    T pn[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl));
     for (int iw = 0; iw < nl; iw++) {
-        T o[4], d[4];
+        T o[4], d[4], pc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) pc[q] = pn[q];
+        if (iw + 1 < nl) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl + iw + 1));
+        }
         HessSink<T> sink{hacc, gacc, d, ivT};
         me_point<T, true>(C, K, T(step7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
             if (s == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    d[q] = o[q] - T(ld<TIn>(prof, base + q * nl + iw));
+                    d[q] = o[q] - pc[q];
                     chi += (double)d[q] * (double)d[q] * (C.wsq[q] * ivn);
                 }
             }
@@ -396,12 +407,21 @@ __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const 
     pix_consts<T>(C, pt.v, K);
     double chi = 0.0;
     FCount fc;
+    T pn[4];   // next wavelength's observed values, loaded one step ahead
+#pragma unroll
+    for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl));
     for (int iw = 0; iw < nl; iw++) {
-        T o[4];
+        T o[4], pc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) pc[q] = pn[q];
+        if (iw + 1 < nl) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl + iw + 1));
+        }
         me_point<T, false>(C, K, T(0), iw, o, NoSink{}, &fc);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const T d = o[q] - T(ld<TIn>(prof, base + q * nl + iw));
+            const T d = o[q] - pc[q];
             chi += (double)d * (double)d * (C.wsq[q] * ivn);
         }
     }
