@@ -345,14 +345,25 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
 #pragma unroll
         for (int q = 0; q < 9; q++) gacc[q] = 0;
         double chi = 0.0;
+        T pn[4] = {T(0), T(0), T(0), T(0)};   // next wavelength's observed values
+        if (l0 < l1) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + l0));
+        }
         for (int iw = l0; iw < l1; iw++) {
-            T o[4], d[4];
+            T o[4], d[4], pc[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) pc[q] = pn[q];
+            if (iw + 1 < l1) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + iw + 1));
+            }
             HessSink<T> sink{hacc, gacc, d, ivT};
             me_point<T, true>(C, K, T(s7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
                 if (s == 0) {
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        d[q] = o[q] - T(ld<TIn>(prof, b + q * nl + iw));
+                        d[q] = o[q] - pc[q];
                         chi += (double)d[q] * (double)d[q] * (C.wsq[q] * iv);
                     }
                 }
