@@ -432,14 +432,19 @@ template <typename T> __device__ __forceinline__ void pix_consts(const DevCfg &C
     K.vterm = T(-1e6 * C.lambda0 / kC * par[3] / par[4]);
 }
 
-template <typename T> __device__ __forceinline__ T reduced_wavelength(const DevCfg &C, const PixK<T> &K, int iw);
-template <> __device__ __forceinline__ double reduced_wavelength<double>(const DevCfg &C, const PixK<double> &K, int iw)
+// the wavelength table entry a precision uses (lambda, or lambda - lambda0
+// for FP32) and the reduced wavelength formed from it
+template <typename T> __device__ __forceinline__ T wavelength_entry(const DevCfg &C, int iw);
+template <> __device__ __forceinline__ double wavelength_entry<double>(const DevCfg &C, int iw) { return C.lam[iw]; }
+template <> __device__ __forceinline__ float wavelength_entry<float>(const DevCfg &C, int iw) { return C.dlamf[iw]; }
+template <typename T> __device__ __forceinline__ T reduced_wavelength_of(const PixK<T> &K, T entry);
+template <> __device__ __forceinline__ double reduced_wavelength_of<double>(const PixK<double> &K, double entry)
 {
-    return 1e3 * C.lam[iw] / K.dw + K.shV;
+    return 1e3 * entry / K.dw + K.shV;
 }
-template <> __device__ __forceinline__ float reduced_wavelength<float>(const DevCfg &C, const PixK<float> &K, int iw)
+template <> __device__ __forceinline__ float reduced_wavelength_of<float>(const PixK<float> &K, float entry)
 {
-    return 1e3f * C.dlamf[iw] / K.dw + K.vterm;
+    return 1e3f * entry / K.dw + K.vterm;
 }
 
 // Stokes vector at wavelength iw; with JAC also the 4x9 Jacobian rows,
@@ -452,12 +457,14 @@ struct NoSink {
     template <typename T> __device__ __forceinline__ void operator()(int, const T *) const {}
 };
 
+// me_point_at takes the wavelength table entry of iw already loaded (the
+// passes load it one wavelength ahead); me_point loads it itself
 template <typename T, bool JAC, typename Sink>
-__device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T step7, int iw, T out[4], Sink &&sink,
-                                         FCount *fcnt)
+__device__ __forceinline__ void me_point_at(const DevCfg &C, const PixK<T> &K, T step7, int iw, T lam_entry,
+                                            T out[4], Sink &&sink, FCount *fcnt)
 {
     const T SP = T(kSqrtPi);
-    const T shW = reduced_wavelength<T>(C, K, iw);
+    const T shW = reduced_wavelength_of<T>(K, lam_entry);
     // derivative slots: 0 = B, 1 = v_los, 2 = Doppler width, 3 = damping
     T ph[3], ps[3], dph[3][4], dps[3][4];
 #pragma unroll
@@ -607,6 +614,13 @@ __device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T st
     row[7] = (-t7 * NV) * step7;
     row[8] = (t8 * NV) * T(C.step[8]);
     sink(3, row);
+}
+
+template <typename T, bool JAC, typename Sink>
+__device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T step7, int iw, T out[4], Sink &&sink,
+                                         FCount *fcnt)
+{
+    me_point_at<T, JAC>(C, K, step7, iw, wavelength_entry<T>(C, iw), out, static_cast<Sink &&>(sink), fcnt);
 }
 
 // ---------------------------------------------------------------- SVD

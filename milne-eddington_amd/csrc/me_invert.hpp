@@ -235,21 +235,24 @@ __device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const vo
     for (int k = 0; k < 9; k++) gacc[k] = 0;
     double chi = 0.0;
     FCount fc;
-    // the observed Stokes values of the next wavelength are loaded while this
-    // one is synthesised, so the loop does not wait on memory each step
-    T pn[4];
+    // the next wavelength's table entry and observed Stokes values are loaded
+    // while this one is synthesised, so the loop does not wait on memory
+    // each step
+    T pn[4], ln = wavelength_entry<T>(C, 0);
 #pragma unroll
     for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl));
     for (int iw = 0; iw < nl; iw++) {
         T o[4], d[4], pc[4];
+        const T lc = ln;
 #pragma unroll
         for (int q = 0; q < 4; q++) pc[q] = pn[q];
         if (iw + 1 < nl) {
+            ln = wavelength_entry<T>(C, iw + 1);
 #pragma unroll
             for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl + iw + 1));
         }
         HessSink<T> sink{hacc, gacc, d, ivT};
-        me_point<T, true>(C, K, T(step7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+        me_point_at<T, true>(C, K, T(step7), iw, lc, o, [&](int s, const T *row) __attribute__((always_inline)) {
             if (s == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -345,21 +348,24 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
 #pragma unroll
         for (int q = 0; q < 9; q++) gacc[q] = 0;
         double chi = 0.0;
-        T pn[4] = {T(0), T(0), T(0), T(0)};   // next wavelength's observed values
+        T pn[4] = {T(0), T(0), T(0), T(0)}, ln = T(0);   // next wavelength's values
         if (l0 < l1) {
+            ln = wavelength_entry<T>(C, l0);
 #pragma unroll
             for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + l0));
         }
         for (int iw = l0; iw < l1; iw++) {
             T o[4], d[4], pc[4];
+            const T lc = ln;
 #pragma unroll
             for (int q = 0; q < 4; q++) pc[q] = pn[q];
             if (iw + 1 < l1) {
+                ln = wavelength_entry<T>(C, iw + 1);
 #pragma unroll
                 for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + iw + 1));
             }
             HessSink<T> sink{hacc, gacc, d, ivT};
-            me_point<T, true>(C, K, T(s7), iw, o, [&](int s, const T *row) __attribute__((always_inline)) {
+            me_point_at<T, true>(C, K, T(s7), iw, lc, o, [&](int s, const T *row) __attribute__((always_inline)) {
                 if (s == 0) {
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
@@ -418,18 +424,21 @@ __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const 
     pix_consts<T>(C, pt.v, K);
     double chi = 0.0;
     FCount fc;
-    T pn[4];   // next wavelength's observed values, loaded one step ahead
+    // next wavelength's table entry and observed values, loaded one step ahead
+    T pn[4], ln = wavelength_entry<T>(C, 0);
 #pragma unroll
     for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl));
     for (int iw = 0; iw < nl; iw++) {
         T o[4], pc[4];
+        const T lc = ln;
 #pragma unroll
         for (int q = 0; q < 4; q++) pc[q] = pn[q];
         if (iw + 1 < nl) {
+            ln = wavelength_entry<T>(C, iw + 1);
 #pragma unroll
             for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, base + q * nl + iw + 1));
         }
-        me_point<T, false>(C, K, T(0), iw, o, NoSink{}, &fc);
+        me_point_at<T, false>(C, K, T(0), iw, lc, o, NoSink{}, &fc);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const T d = o[q] - pc[q];
