@@ -302,8 +302,8 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
 // halves' partial sums (each in wavelength order), whichever lanes compute
 // them, so results do not depend on how pixels fall into waves.  The lanes
 // that need a normal matrix (`own`) are ranked r = 0 .. nJ-1; work item
-// i = h nJ + r (half h of pixel r) goes to lane i mod 64 in round i div 64:
-// one round of nl/2 wavelengths while nJ <= 32, instead of nl per lane.
+// i = h nJ + r (half h of pixel r) goes to lane i mod 64 in batch i div 64:
+// one batch of nl/2 wavelengths while nJ <= 32, instead of nl per lane.
 // Owners add the halves in order and write them like jac_pass.  Returns the
 // owner's chi2 and this lane's own Faddeeva branch counts.  Called by the
 // whole wave.
@@ -319,7 +319,7 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
     const int nJ = __popcll(mask);
     const int lane = __lane_id();
     const int rk = __popcll(mask & lanemask_lt());            // owner's rank
-    const int rounds = (nh * nJ + 63) / 64;
+    const int batches = (nh * nJ + 63) / 64;
     T H[45], G[9];
 #pragma unroll
     for (int q = 0; q < 45; q++) H[q] = 0;
@@ -327,7 +327,7 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
     for (int q = 0; q < 9; q++) G[q] = 0;
     double chs = 0.0;
     FCount fc;
-    for (int t = 0; t < rounds; t++) {
+    for (int t = 0; t < batches; t++) {
         const int i = 64 * t + lane;
         const int h = i / nJ, r = i % nJ;
         const bool work = i < nh * nJ;
@@ -376,8 +376,8 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
                 sink(s, row);
             }, &fc);
         }
-        // the owner of rank rk takes half 0 (item rk, always in round 0) and
-        // then half 1 (item nJ + rk) when they fall into this round
+        // the owner of rank rk takes half 0 (item rk, always in batch 0) and
+        // then half 1 (item nJ + rk) when they fall into this batch
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
             const int it = hh * nJ + rk;
