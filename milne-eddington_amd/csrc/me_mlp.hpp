@@ -414,8 +414,16 @@ __global__ __launch_bounds__(MI_T) void mlp_infer_kernel(const TIn *__restrict__
             const int KB = d.KB[l], T = d.T[l];
             __syncthreads();
             const bf16x8 *src = wpk + d.frag[l] * 64;
-            for (int i = tid; i < T * KB * 64; i += MI_T) sW[i] = src[i];
+            // the layer's fragments go straight from global memory to LDS
+            // (global_load_lds: no VGPR round trip, every load of the wave in
+            // flight at once); a wave copies 64 consecutive fragments per
+            // instruction, and fragment counts are multiples of 64
+            const int nfr = T * KB * 64;
+            for (int i0 = wave * 64; i0 < nfr; i0 += MI_T)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + i0 + lane),
+                                                 (__attribute__((address_space(3))) void *)(sW + i0), 16, 0, 0);
             for (int i = tid; i < 256; i += MI_T) sb[i] = bpk[l * 256 + i];
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (l < MLP_NL - 1) {
                 if (KB == 8) mlp_hidden_layer<8>(sW, sb, lane, g, act);
