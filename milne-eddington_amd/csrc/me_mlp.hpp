@@ -26,6 +26,7 @@ constexpr int MLP_H = 256;      // hidden width
 constexpr int MLP_OUT = 10;     // 9 parameters with phi as (sin 2phi, cos 2phi)
 constexpr int MLP_OUTP = 16;    // output width padded for the MFMA tile
 constexpr int MLP_NL = 4;       // layers
+constexpr int MLP_KSPLIT = 32;  // batch (K) chunks of a weight-gradient GEMM
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return __float2bfloat16(x); }
 __device__ __forceinline__ float from_bf16(__bf16 x) { return __bfloat162float(x); }
@@ -60,7 +61,9 @@ enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
 //   EPI_FWD_GELU    Z = C + bias (fp32 out), Y = gelu(Z) (bf16 out)
 //   EPI_FWD_LINEAR  Z = C + bias (fp32 out)
 //   EPI_BWD_GELU    Y = C * gelu'(aux) (bf16 out; aux = pre-activation fp32)
-//   EPI_F32         Z += C * scale (fp32 atomics; K split over gridDim.z)
+//   EPI_F32         Z[z] = C * scale, the partial product of K chunk z =
+//                   blockIdx.z (K split over gridDim.z); Z[z] starts at
+//                   z * M * ldc, and sum_partials adds them in chunk order
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
                                                  const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
             } else if (EPI == EPI_BWD_GELU) {
                 Yb[o] = to_bf16(c * gelu_grad(aux[o]));
             } else {
-                atomicAdd(&Zf[o], c * scale);  // split-K partial sums (Zf zeroed by the caller)
+                Zf[(int64_t)blockIdx.z * M * ldc + o] = c * scale;  // this K chunk's partial
             }
         }
     }
@@ -229,21 +232,36 @@ __global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ 
 // column sums of a bf16 [n][w] matrix (bias gradients): 256 threads = 64
 // columns x 4 row groups per block, blockIdx.y selects a chunk of rows;
 // partial sums go to out with atomics (out zeroed by the caller)
+// partial column sums of one chunk of rows (blockIdx.y) into part[chunk][w];
+// wave g takes rows g, g + 4, ... of the chunk and the four partials are added
+// in wave order.  sum_partials then adds the chunks in order: the bias
+// gradients come out the same in every run (no atomics)
 __global__ __launch_bounds__(256) void col_sum(const __bf16 *__restrict__ D, int n, int w, int rows_per_chunk,
-                                               float *__restrict__ out)
+                                               float *__restrict__ part)
 {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int g = threadIdx.x >> 6;
     const int r0 = blockIdx.y * rows_per_chunk;
     const int r1 = n < r0 + rows_per_chunk ? n : r0 + rows_per_chunk;
-    __shared__ float part[4][64];
+    __shared__ float ps[4][64];
     float s = 0.f;
     if (c < w)
         for (int r = r0 + g; r < r1; r += 4) s += from_bf16(D[(int64_t)r * w + c]);
-    part[g][threadIdx.x & 63] = s;
+    ps[g][threadIdx.x & 63] = s;
     __syncthreads();
-    if (g == 0 && c < w) atomicAdd(&out[c], part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                                              part[3][threadIdx.x]);
+    if (g == 0 && c < w)
+        part[(int64_t)blockIdx.y * w + c] = ps[0][threadIdx.x] + ps[1][threadIdx.x] + ps[2][threadIdx.x] + ps[3][threadIdx.x];
+}
+
+// out[i] = sum over z of part[z][i], z = 0 .. nz-1 in order
+__global__ __launch_bounds__(256) void sum_partials(const float *__restrict__ part, int nz, int64_t n,
+                                                    float *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = part[i];
+    for (int z = 1; z < nz; z++) s += part[(int64_t)z * n + i];
+    out[i] = s;
 }
 
 // Adam on FP32 masters, then the bf16 copy used by the GEMMs
@@ -476,6 +494,7 @@ struct MlpState {
     int dims[MLP_NL + 1] = {0, MLP_H, MLP_H, MLP_H, MLP_OUTP};
     // parameters: for each layer W [din][dout] then b [dout], FP32 masters
     float *P = nullptr, *Mo = nullptr, *Vo = nullptr, *G = nullptr;
+    float *Gp = nullptr;   // split-K partials of one layer's weight gradient
     __bf16 *Pb = nullptr;
     int64_t np = 0;
     int64_t offW[MLP_NL] = {0}, offB[MLP_NL] = {0};
@@ -521,7 +540,7 @@ inline int64_t mlp_num_params(const MlpState &m)
 
 inline void mlp_free(MlpState &m)
 {
-    dev_release(m.P); dev_release(m.Mo); dev_release(m.Vo); dev_release(m.G); dev_release(m.Pb);
+    dev_release(m.P); dev_release(m.Mo); dev_release(m.Vo); dev_release(m.G); dev_release(m.Gp); dev_release(m.Pb);
     dev_release(m.X); dev_release(m.Y); dev_release(m.loss); dev_release(m.prof_tmp); dev_release(m.truth_tmp);
     dev_release(m.wpk); dev_release(m.bpk);
     for (int l = 0; l < MLP_NL; l++) { dev_release(m.A[l]); dev_release(m.D[l]); dev_release(m.Z[l]); }
@@ -550,6 +569,10 @@ inline me_status mlp_alloc_params(MlpState &m)
     e = e ? e : hipMalloc(&m.Mo, sizeof(float) * off);
     e = e ? e : hipMalloc(&m.Vo, sizeof(float) * off);
     e = e ? e : hipMalloc(&m.G, sizeof(float) * off);
+    int64_t wmax = 0;
+    for (int l = 0; l < MLP_NL; l++)
+        wmax = wmax > (int64_t)m.dims[l] * m.dims[l + 1] ? wmax : (int64_t)m.dims[l] * m.dims[l + 1];
+    e = e ? e : hipMalloc(&m.Gp, sizeof(float) * MLP_KSPLIT * wmax);
     e = e ? e : hipMalloc(&m.Pb, sizeof(__bf16) * off);
     e = e ? e : hipMalloc(&m.loss, sizeof(float));
     return e == hipSuccess ? ME_OK : ME_ENOMEM;
@@ -623,10 +646,8 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
 // backward pass (loss gradient already in m.D[last]); gradients into m.G
 inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
 {
-    // gradients accumulate with atomics: clear them with a fill kernel
-    hipLaunchKernelGGL(fill_f32, dim3((m.np + 255) / 256), dim3(256), 0, s, m.G, m.np, 0.0f);
-    const int ksplit = rows >= 4096 ? 32 : (rows >= 1024 ? 8 : 1);
-    const int rchunk = (rows + 63) / 64;
+    // every gradient entry is written once, in a fixed summation order
+    const int ksplit = rows >= 4096 ? MLP_KSPLIT : (rows >= 1024 ? 8 : 1);
     for (int l = MLP_NL - 1; l >= 0; l--) {
         const int din = m.dims[l], dout = m.dims[l + 1];
         const __bf16 *act = l == 0 ? m.X : m.A[l - 1];
@@ -634,10 +655,18 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
         dim3 g = gemm_grid(din, dout);
         g.z = ksplit;
         hipLaunchKernelGGL(gemm_bf16<EPI_F32>, g, dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
-                           (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.G + m.offW[l],
+                           (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.Gp,
                            (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f);
-        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, (rows + rchunk - 1) / rchunk), dim3(256), 0, s,
-                           (const __bf16 *)m.D[l], rows, dout, rchunk, m.G + m.offB[l]);
+        const int64_t nw = (int64_t)din * dout;
+        hipLaunchKernelGGL(sum_partials, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
+                           ksplit, nw, m.G + m.offW[l]);
+        // bias gradients: row-chunk partials (in Gp, free again once the
+        // weight gradient is summed), then added in chunk order
+        const int rchunk = (rows + 63) / 64, nch = (rows + rchunk - 1) / rchunk;
+        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, nch), dim3(256), 0, s, (const __bf16 *)m.D[l], rows, dout,
+                           rchunk, m.Gp);
+        hipLaunchKernelGGL(sum_partials, dim3((unsigned)((dout + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
+                           nch, (int64_t)dout, m.G + m.offB[l]);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
             hipLaunchKernelGGL(gemm_bf16<EPI_BWD_GELU>, gemm_grid(rows, din), dim3(256), 0, s,
