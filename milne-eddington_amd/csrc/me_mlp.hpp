@@ -63,13 +63,13 @@ enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
 //   EPI_BWD_GELU    Y = C * gelu'(aux) (bf16 out; aux = pre-activation fp32)
 //   EPI_F32         Z[z] = C * scale, the partial product of K chunk z =
 //                   blockIdx.z (K split over gridDim.z); Z[z] starts at
-//                   z * M * ldc, and sum_partials adds them in chunk order
+//                   z * zs, and sum_partials adds them in chunk order
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
                                                  const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
                                                  int K, float *__restrict__ Zf, __bf16 *__restrict__ Yb, int ldc,
                                                  const float *__restrict__ bias, const float *__restrict__ aux,
-                                                 float scale)
+                                                 float scale, int64_t zs)
 {
     __shared__ __bf16 sA[64][32 + 8];
     __shared__ __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
             } else if (EPI == EPI_BWD_GELU) {
                 Yb[o] = to_bf16(c * gelu_grad(aux[o]));
             } else {
-                Zf[(int64_t)blockIdx.z * M * ldc + o] = c * scale;  // this K chunk's partial
+                Zf[(int64_t)blockIdx.z * zs + o] = c * scale;  // this K chunk's partial
             }
         }
     }
@@ -209,9 +209,11 @@ __global__ void mlp_decode(const float *__restrict__ Z, const TIn *__restrict__ 
     }
 }
 
-// loss = mean over rows and the 10 outputs of (Z - Y)^2; dZ = 2 (Z - Y) / (n*10)
+// loss = mean over rows and the 10 outputs of (Z - Y)^2; dZ = 2 (Z - Y) / (n*10).
+// Each wave writes its share of the loss to loss_part[wave] (4 waves per
+// block of 256 rows); mlp_loss_total adds them in wave order
 __global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ Y, int n, __bf16 *__restrict__ dZ,
-                         float *__restrict__ loss_acc)
+                         float *__restrict__ loss_part)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.f;
@@ -226,18 +228,17 @@ __global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ 
         }
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss_acc, l / (float)(n * MLP_OUT));
+    if ((threadIdx.x & 63) == 0) loss_part[blockIdx.x * 4 + (threadIdx.x >> 6)] = l / (float)(n * MLP_OUT);
 }
 
 // column sums of a bf16 [n][w] matrix (bias gradients): 256 threads = 64
-// columns x 4 row groups per block, blockIdx.y selects a chunk of rows;
-// partial sums go to out with atomics (out zeroed by the caller)
-// partial column sums of one chunk of rows (blockIdx.y) into part[chunk][w];
-// wave g takes rows g, g + 4, ... of the chunk and the four partials are added
-// in wave order.  sum_partials then adds the chunks in order: the bias
-// gradients come out the same in every run (no atomics)
+// columns x 4 row groups per block; blockIdx.y selects a chunk of rows and
+// its partial column sums go to part[chunk * zs + column] (a chunk past the
+// last row writes zeros).  Wave g takes rows g, g + 4, ... of the chunk and
+// the four partials are added in wave order; sum_partials then adds the
+// chunks in order, so the bias gradients come out the same in every run
 __global__ __launch_bounds__(256) void col_sum(const __bf16 *__restrict__ D, int n, int w, int rows_per_chunk,
-                                               float *__restrict__ part)
+                                               float *__restrict__ part, int64_t zs)
 {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int g = threadIdx.x >> 6;
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(256) void col_sum(const __bf16 *__restrict__ D, int
     ps[g][threadIdx.x & 63] = s;
     __syncthreads();
     if (g == 0 && c < w)
-        part[(int64_t)blockIdx.y * w + c] = ps[0][threadIdx.x] + ps[1][threadIdx.x] + ps[2][threadIdx.x] + ps[3][threadIdx.x];
+        part[(int64_t)blockIdx.y * zs + c] = ps[0][threadIdx.x] + ps[1][threadIdx.x] + ps[2][threadIdx.x] + ps[3][threadIdx.x];
 }
 
 // out[i] = sum over z of part[z][i], z = 0 .. nz-1 in order
@@ -494,7 +495,7 @@ struct MlpState {
     int dims[MLP_NL + 1] = {0, MLP_H, MLP_H, MLP_H, MLP_OUTP};
     // parameters: for each layer W [din][dout] then b [dout], FP32 masters
     float *P = nullptr, *Mo = nullptr, *Vo = nullptr, *G = nullptr;
-    float *Gp = nullptr;   // split-K partials of one layer's weight gradient
+    float *Gp = nullptr;   // [MLP_KSPLIT][np] chunk partials of the gradient
     __bf16 *Pb = nullptr;
     int64_t np = 0;
     int64_t offW[MLP_NL] = {0}, offB[MLP_NL] = {0};
@@ -569,22 +570,23 @@ inline me_status mlp_alloc_params(MlpState &m)
     e = e ? e : hipMalloc(&m.Mo, sizeof(float) * off);
     e = e ? e : hipMalloc(&m.Vo, sizeof(float) * off);
     e = e ? e : hipMalloc(&m.G, sizeof(float) * off);
-    int64_t wmax = 0;
-    for (int l = 0; l < MLP_NL; l++)
-        wmax = wmax > (int64_t)m.dims[l] * m.dims[l + 1] ? wmax : (int64_t)m.dims[l] * m.dims[l + 1];
-    e = e ? e : hipMalloc(&m.Gp, sizeof(float) * MLP_KSPLIT * wmax);
+    e = e ? e : hipMalloc(&m.Gp, sizeof(float) * MLP_KSPLIT * off);
     e = e ? e : hipMalloc(&m.Pb, sizeof(__bf16) * off);
-    e = e ? e : hipMalloc(&m.loss, sizeof(float));
     return e == hipSuccess ? ME_OK : ME_ENOMEM;
 }
+
+// waves of one mlp_loss launch over a batch of rows; m.loss holds their
+// partials followed by the total
+inline int mlp_loss_waves(int64_t rows) { return (int)((rows + 255) / 256) * 4; }
 
 inline me_status mlp_alloc_acts(MlpState &m, int64_t rows)
 {
     if (rows <= m.cap) return ME_OK;
     m.cap = 0;
-    dev_release(m.X); dev_release(m.Y);
+    dev_release(m.X); dev_release(m.Y); dev_release(m.loss);
     for (int l = 0; l < MLP_NL; l++) { dev_release(m.A[l]); dev_release(m.D[l]); dev_release(m.Z[l]); }
     hipError_t e = hipMalloc(&m.X, sizeof(__bf16) * rows * m.ninp);
+    e = e ? e : hipMalloc(&m.loss, sizeof(float) * (mlp_loss_waves(rows) + 1));
     e = e ? e : hipMalloc(&m.Y, sizeof(float) * rows * MLP_OUTP);
     for (int l = 0; l < MLP_NL; l++) {
         const int w = m.dims[l + 1];
@@ -634,11 +636,11 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
         if (l < MLP_NL - 1)
             hipLaunchKernelGGL(gemm_bf16<EPI_FWD_GELU>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
                                (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], m.A[l], dout, b,
-                               (const float *)nullptr, 1.0f);
+                               (const float *)nullptr, 1.0f, (int64_t)0);
         else
             hipLaunchKernelGGL(gemm_bf16<EPI_FWD_LINEAR>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
                                (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], (__bf16 *)nullptr,
-                               dout, b, (const float *)nullptr, 1.0f);
+                               dout, b, (const float *)nullptr, 1.0f, (int64_t)0);
         in = m.A[l];
     }
 }
@@ -646,7 +648,10 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
 // backward pass (loss gradient already in m.D[last]); gradients into m.G
 inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
 {
-    // every gradient entry is written once, in a fixed summation order
+    // every gradient entry is the sum of ksplit chunk partials (weights: the
+    // K chunks of the GEMM; biases: row chunks), which sit in Gp at stride np
+    // at the entry's own offset; one sum_partials adds them all in chunk
+    // order at the end, so the gradient is the same in every run
     const int ksplit = rows >= 4096 ? MLP_KSPLIT : (rows >= 1024 ? 8 : 1);
     for (int l = MLP_NL - 1; l >= 0; l--) {
         const int din = m.dims[l], dout = m.dims[l + 1];
@@ -655,26 +660,32 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
         dim3 g = gemm_grid(din, dout);
         g.z = ksplit;
         hipLaunchKernelGGL(gemm_bf16<EPI_F32>, g, dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
-                           (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.Gp,
-                           (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f);
-        const int64_t nw = (int64_t)din * dout;
-        hipLaunchKernelGGL(sum_partials, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
-                           ksplit, nw, m.G + m.offW[l]);
-        // bias gradients: row-chunk partials (in Gp, free again once the
-        // weight gradient is summed), then added in chunk order
-        const int rchunk = (rows + 63) / 64, nch = (rows + rchunk - 1) / rchunk;
-        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, nch), dim3(256), 0, s, (const __bf16 *)m.D[l], rows, dout,
-                           rchunk, m.Gp);
-        hipLaunchKernelGGL(sum_partials, dim3((unsigned)((dout + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
-                           nch, (int64_t)dout, m.G + m.offB[l]);
+                           (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.Gp + m.offW[l],
+                           (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f, m.np);
+        // db = column sums of dZ over ksplit row chunks
+        const int rchunk = (rows + ksplit - 1) / ksplit;
+        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, ksplit), dim3(256), 0, s, (const __bf16 *)m.D[l], rows,
+                           dout, rchunk, m.Gp + m.offB[l], m.np);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
             hipLaunchKernelGGL(gemm_bf16<EPI_BWD_GELU>, gemm_grid(rows, din), dim3(256), 0, s,
                                (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, (const __bf16 *)(m.Pb + m.offW[l]),
                                (int64_t)1, (int64_t)dout, rows, din, dout, (float *)nullptr, m.D[l - 1], din,
-                               (const float *)nullptr, (const float *)m.Z[l - 1], 1.0f);
+                               (const float *)nullptr, (const float *)m.Z[l - 1], 1.0f, (int64_t)0);
         }
     }
+    hipLaunchKernelGGL(sum_partials, dim3((unsigned)((m.np + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
+                       ksplit, m.np, m.G);
+}
+
+// the batch loss of the last mlp_loss launch (its wave partials added in
+// order, so the same batch gives the same value in every run)
+inline hipError_t mlp_loss_total(MlpState &m, hipStream_t s, int rows, float *host)
+{
+    const int nw = mlp_loss_waves(rows);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, s, (const float *)m.loss, nw, (int64_t)1, m.loss + nw);
+    hipError_t e = hipMemcpyAsync(host, m.loss + nw, sizeof(float), hipMemcpyDeviceToHost, s);
+    return e ? e : hipStreamSynchronize(s);
 }
 
 inline me_status mlp_tmp(MlpState &m, int64_t rows, int nl)
@@ -719,7 +730,6 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
                            (const double *)(m.truth_tmp + r0 * 9), (const float *)(m.prof_tmp + r0 * 4 * nl),
                            (int64_t)batch, nl, m.norm, m.Y);
         mlp_forward(m, s, batch);
-        hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, s, m.loss, (int64_t)1, 0.0f);
         hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
                            (const float *)m.Y, batch, m.D[MLP_NL - 1], m.loss);
         mlp_backward(m, s, batch);
@@ -731,9 +741,7 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
                            m.Pb, m.np, lrs, b1, b2, 1e-8f, c1, c2);
         if (step == total - 1) {
             float lh = 0;
-            if (hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess)
-                return ME_EDEVICE;
+            if (mlp_loss_total(m, s, batch, &lh) != hipSuccess) return ME_EDEVICE;
             lossv = lh;
         }
     }
@@ -878,7 +886,6 @@ inline me_status mlp_grad_host(MlpState &m, hipStream_t s, const float *x, const
     }
     hipError_t e = hipMemcpyAsync(m.X, hx.data(), sizeof(__bf16) * hx.size(), hipMemcpyHostToDevice, s);
     e = e ? e : hipMemcpyAsync(m.Y, hy.data(), sizeof(float) * hy.size(), hipMemcpyHostToDevice, s);
-    e = e ? e : hipMemsetAsync(m.loss, 0, sizeof(float), s);
     if (e != hipSuccess) return ME_EDEVICE;
     mlp_forward(m, s, batch);
     hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
@@ -887,8 +894,7 @@ inline me_status mlp_grad_host(MlpState &m, hipStream_t s, const float *x, const
     std::vector<float> hg(m.np);
     float lh = 0;
     e = hipMemcpyAsync(hg.data(), m.G, sizeof(float) * m.np, hipMemcpyDeviceToHost, s);
-    e = e ? e : hipMemcpyAsync(&lh, m.loss, sizeof(float), hipMemcpyDeviceToHost, s);
-    e = e ? e : hipStreamSynchronize(s);
+    e = e ? e : mlp_loss_total(m, s, batch, &lh);
     if (e != hipSuccess) return ME_EDEVICE;
     mlp_for_each_exported(m, [&](int64_t ei, int64_t i) { grad[ei] = hg[i]; });
     if (loss) *loss = lh;

@@ -223,16 +223,19 @@ def test_imported_network_predicts_like_the_trained_one():
 
 
 def test_training_is_bit_reproducible():
-    """Two trainings with the same seed and settings give the same network:
-    the weight and bias gradients are summed in a fixed order (split-K
-    partials added in chunk order, column sums without atomics)."""
+    """Two trainings with the same seed and settings give the same network
+    and report the same final loss: the weight and bias gradients and the
+    batch loss are summed in a fixed order (split-K partials added in chunk
+    order, column sums and loss partials without atomics)."""
     w = get("hmi4k")
     ranges = w.ranges()
-    nets = []
+    nets, losses = [], []
     for _ in range(2):
         with gpu_inversion("hmi4k", precision=ME_FP32) as inv:
-            inv.mlp_train(w.seed + 11, ranges, 65536, batch=4096, epochs=2, lr=2e-3, ff_frac=w.ff_frac,
-                          ff_bmax=w.ff_bmax, sigma=w.sigma)
+            losses.append(inv.mlp_train(w.seed + 11, ranges, 65536, batch=4096, epochs=2, lr=2e-3,
+                                        ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma))
             nets.append(inv.mlp_params())
     assert np.all(np.isfinite(nets[0]))
     np.testing.assert_array_equal(nets[1], nets[0])
+    assert np.isfinite(losses[0]) and losses[0] > 0
+    assert losses[1] == losses[0]
