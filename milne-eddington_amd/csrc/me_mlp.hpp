@@ -231,27 +231,32 @@ __global__ void mlp_loss(const float *__restrict__ Z, const float *__restrict__ 
     if ((threadIdx.x & 63) == 0) loss_part[blockIdx.x * 4 + (threadIdx.x >> 6)] = l / (float)(n * MLP_OUT);
 }
 
-// column sums of a bf16 [n][w] matrix (bias gradients): 256 threads = 64
-// columns x 4 row groups per block; blockIdx.y selects a chunk of rows and
-// its partial column sums go to part[chunk * zs + column] (a chunk past the
-// last row writes zeros).  Wave g takes rows g, g + 4, ... of the chunk and
-// the four partials are added in wave order; sum_partials then adds the
-// chunks in order, so the bias gradients come out the same in every run
-__global__ __launch_bounds__(256) void col_sum(const __bf16 *__restrict__ D, int n, int w, int rows_per_chunk,
-                                               float *__restrict__ part, int64_t zs)
+// column sums of a bf16 [n][w] matrix (bias gradients): COL_SUM_G waves =
+// 64 columns x COL_SUM_G row groups per block; blockIdx.y selects a chunk of
+// rows and its partial column sums go to part[chunk * zs + column] (a chunk
+// past the last row writes zeros).  Wave g takes rows g, g + COL_SUM_G, ...
+// of the chunk and the group partials are added in wave order; sum_partials
+// then adds the chunks in order, so the bias gradients come out the same in
+// every run
+constexpr int COL_SUM_G = 16;
+__global__ __launch_bounds__(64 * COL_SUM_G) void col_sum(const __bf16 *__restrict__ D, int n, int w,
+                                                          int rows_per_chunk, float *__restrict__ part, int64_t zs)
 {
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int g = threadIdx.x >> 6;
     const int r0 = blockIdx.y * rows_per_chunk;
     const int r1 = n < r0 + rows_per_chunk ? n : r0 + rows_per_chunk;
-    __shared__ float ps[4][64];
+    __shared__ float ps[COL_SUM_G][64];
     float s = 0.f;
     if (c < w)
-        for (int r = r0 + g; r < r1; r += 4) s += from_bf16(D[(int64_t)r * w + c]);
+        for (int r = r0 + g; r < r1; r += COL_SUM_G) s += from_bf16(D[(int64_t)r * w + c]);
     ps[g][threadIdx.x & 63] = s;
     __syncthreads();
-    if (g == 0 && c < w)
-        part[(int64_t)blockIdx.y * zs + c] = ps[0][threadIdx.x] + ps[1][threadIdx.x] + ps[2][threadIdx.x] + ps[3][threadIdx.x];
+    if (g == 0 && c < w) {
+        float t = ps[0][threadIdx.x];
+        for (int k = 1; k < COL_SUM_G; k++) t += ps[k][threadIdx.x];
+        part[(int64_t)blockIdx.y * zs + c] = t;
+    }
 }
 
 // out[i] = sum over z of part[z][i], z = 0 .. nz-1 in order
@@ -280,12 +285,6 @@ __global__ void adam_step(float *__restrict__ p, float *__restrict__ m, float *_
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
     if (pb) pb[i] = to_bf16(pi);
-}
-
-__global__ void fill_f32(float *__restrict__ x, int64_t n, float v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__ y, int64_t n)
@@ -664,7 +663,7 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
                            (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f, m.np);
         // db = column sums of dZ over ksplit row chunks
         const int rchunk = (rows + ksplit - 1) / ksplit;
-        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, ksplit), dim3(256), 0, s, (const __bf16 *)m.D[l], rows,
+        hipLaunchKernelGGL(col_sum, dim3((dout + 63) / 64, ksplit), dim3(64 * COL_SUM_G), 0, s, (const __bf16 *)m.D[l], rows,
                            dout, rchunk, m.Gp + m.offB[l], m.np);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
