@@ -62,7 +62,7 @@ typedef struct me_config {
     double lambda_reject;      /* 5                                         */
     double lambda_limits[2];   /* 1e-5, 1e3                                 */
     double bounds[9][2];       /* Bounds_*; [1],[2] in rad (never applied)  */
-    int    fixed[9];           /* *_fixed (FP64 oracle only; the device path needs all 9 free) */
+    int    fixed[9];           /* *_fixed: parameter held at its starting value (npar < 9) */
     int    regul[9];           /* *_Regularization flag                     */
     double regul_value[9];     /* constant (theta, phi already x pi)        */
     double regul_weight[9];    /* weight                                    */
@@ -120,6 +120,15 @@ void me_config_defaults(me_config *cfg);
  */
 me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_ctx **out);
 void      me_ctx_destroy(me_ctx *ctx);
+
+/* Global index of the first pixel of the next batches (default 0).  Restart
+ * jumps at irun >= 4 draw from Philox4x32-10 with counter (base + index of the
+ * pixel in the batch, run, stream), so a pixel gets the same draws whichever
+ * batch it is inverted in: a host that splits a map into batches, or resumes
+ * an interrupted run, passes each batch's offset in the map.  Replaces the
+ * reference's time-seeded RNG state (Src/RANDOM_NUMBER.c:25-61, used by
+ * Random_Jump, Src/LM_FIT.c:185-197), which is not reproducible at all. */
+me_status me_ctx_set_pixel_base(me_ctx *ctx, int64_t base);
 
 /* Derived constants of the context: ic, i0, i1, ncut and v_los bounds. */
 me_status me_ctx_info(const me_ctx *ctx, int *ic, int *i0, int *i1, int *ncut, double vlos_bounds[2]);

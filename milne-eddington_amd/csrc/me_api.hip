@@ -40,6 +40,7 @@ struct DevBuf {
 struct me_ctx {
     me_config cfg;
     int nl = 0, device = 0, ncu = 0;
+    int64_t pix_base = 0;  // me_ctx_set_pixel_base
     DevCfg dc;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // inversion kernel
@@ -133,6 +134,7 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
     me::InvArgs A;
     A.prof = d_prof;
     A.n = n;
+    A.pix_base = c->pix_base;
     A.res = d_res;
     A.fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
     A.init = d_init;
@@ -446,6 +448,13 @@ me_status me_ctx_set_region_counts(me_ctx *c, int on)
     c->dc.fregions = on ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->d_cfg, &c->dc, sizeof(DevCfg), hipMemcpyHostToDevice));
+    return ME_OK;
+}
+
+me_status me_ctx_set_pixel_base(me_ctx *c, int64_t base)
+{
+    if (!c || base < 0) return ME_EINVAL;
+    c->pix_base = base;
     return ME_OK;
 }
 

@@ -31,6 +31,7 @@ enum { SL_G = 0, SL_MAT = 9, SL_N = 9 + 162 };
 struct InvArgs {
     const void *prof;
     int64_t n;
+    int64_t pix_base;  // global index of pixel 0 of the batch: the Philox counter of its restart draws
     double *res;
     double *fitpar;  // [n][9] parameters of the best fit before HMI_REF (fit profiles are synthesised from them)
     const double *init;
@@ -802,7 +803,8 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             bool finish = false, restart = false;
             if (lphase == 1) {
                 // learned start: accept at chisq_criteria, else heuristic runs
-                if (crun <= C.crit) {
+                // (none when num_run = 0: the learned record is kept)
+                if (crun <= C.crit || C.nruns == 0) {
 #pragma unroll
                     for (int k = 0; k < 9; k++) SCR(SL_BEST + k) = par[k];
                     SCR(SL_CHI_BEST) = crun;
@@ -852,8 +854,8 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                     par[2] = SCR(SL_GUESS + 2) + kPi / 4. * r;
                 } else {
                     double nz[8];
-                    gauss4(C.seed, pix, (uint32_t)r, kStreamJump + 0, nz);
-                    gauss4(C.seed, pix, (uint32_t)r, kStreamJump + 1, nz + 4);
+                    gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 0, nz);
+                    gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 1, nz + 4);
                     par[0] = SCR(SL_GUESS + 0) * (1 + 0.6 * nz[0]);
                     par[1] = SCR(SL_GUESS + 1) + kPi / 8. * nz[1];
                     par[2] = SCR(SL_GUESS + 2) + kPi / 4. * (r + 0.3 * nz[2]);

@@ -367,8 +367,11 @@ int invert_cube(const mehost::Control &ctl_in, const Options &opt, const Paths &
         }
         me_stats stats;
         std::memset(&stats, 0, sizeof(stats));
-        me_status st =
-            me_invert_batch_f64(ctx, prof.data(), n, res.data(), output_fit ? fit.data() : nullptr, nullptr, &stats);
+        // restart draws are keyed by the pixel's index in the box, so a
+        // resumed run gives every pixel the draws of an uninterrupted one
+        me_status st = me_ctx_set_pixel_base(ctx, p0);
+        if (st != ME_OK) return fail("me_ctx_set_pixel_base failed (status " + std::to_string((int)st) + ")");
+        st = me_invert_batch_f64(ctx, prof.data(), n, res.data(), output_fit ? fit.data() : nullptr, nullptr, &stats);
         if (st != ME_OK) return fail("me_invert_batch_f64 failed (status " + std::to_string((int)st) + ")");
         kernel_ms += stats.kernel_ms;
         if (!mefits::write_pixels(res_img, p0 * 10, n * 10, res.data(), err)) return fail(err);

@@ -87,6 +87,7 @@ ENTRY_POINTS = [
     ("me_ctx_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                               C.POINTER(C.c_int), _dp]),
     ("me_ctx_set_region_counts", C.c_int, [_vp, C.c_int]),
+    ("me_ctx_set_pixel_base", C.c_int, [_vp, _i64]),
     ("me_ctx_autotune", C.c_int, [_vp, _vp, C.c_int, _i64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),
     ("me_invert_batch", C.c_int, [_vp, _fp, _i64, _dp, _dp, _dp, C.POINTER(MeStats)]),

@@ -92,6 +92,11 @@ class Inversion:
         (n_fadd); off saves a few per cent of kernel time."""
         _check(self.lib.me_ctx_set_region_counts(self._h, int(bool(on))), "me_ctx_set_region_counts")
 
+    def set_pixel_base(self, base: int):
+        """Global index of the first pixel of the following batches: the
+        Philox counter of the restart jumps is base + index in the batch."""
+        _check(self.lib.me_ctx_set_pixel_base(self._h, int(base)), "me_ctx_set_pixel_base")
+
     def autotune(self, d_prof: int, n: int, prof_is_f64: bool = False):
         """Pick the inversion kernel's launch shape on the first n device
         profiles (me_ctx_autotune); the choice stays in this context."""

@@ -1041,7 +1041,7 @@ typedef struct orc_work {
     double pbuf[3][9];
     double *P, *Pb, *Pt;
     double Pg[9];
-    double *syn, *fit, *fitb, *jac, *prof;
+    double *syn, *fit, *fitb, *fitl, *jac, *prof;
     double H[81], V[81], W[9], g[9], sol[9];
     double damp;
     double lm_chisq;      /* H2: last trial chi2 of the previous LM_FIT */
@@ -1172,51 +1172,66 @@ static double orc_trial(orc_work *W)
     }
     orc_bounds_check(W->Pt, (const double(*)[2])W->lim);
     orc_synth_w(W, W->Pt, 0);
-    W->trials++;
-    W->lm_chisq = orc_loss(W);
+    const double chi = orc_loss(W);
     if (W->c->regul_any) orc_regul(W, W->Pt, 0);
-    return W->lm_chisq;
+    return chi;
 }
 
 #define ORC_SWAP(a, b) do { double *t_ = (a); (a) = (b); (b) = t_; } while (0)
 
-/* one LM run (Src/LM_FIT.c:531-661); returns the iteration count */
-static int orc_lm_fit(orc_work *W)
+/* The control of one LM run (Src/LM_FIT.c:531-661) over callbacks that do
+ * the arithmetic: the oracle's LM (orc_lm_fit) and the replay hook of the
+ * tests (orc_lm_replay) run this same function, so the trial count under the
+ * damping bound (Lambda_propose, :124-153), the damping reset after every
+ * iteration (:633), the accept test (:591), the regularisation ratio updates
+ * (:557-560, :596-598), the convergence test (:57-58, :638-642) and the
+ * num_iter cap are checked on constructed chi2 sequences. */
+typedef struct orc_lm_ops {
+    void *u;
+    double (*start)(void *u);                /* synthesis + Jacobian at P: chi2      */
+    double (*penalty_p)(void *u);            /* Get_Regul(P) without the matrix     */
+    void (*normal)(void *u, double ratio);   /* Hessian_Compute + Get_Regul(matrix) */
+    double (*trial)(void *u, double *pen);   /* Trial_Synthesis: chi2, penalty(Pt)   */
+    void (*accept)(void *u);                 /* P <- Pt, synthesis + Jacobian at P  */
+} orc_lm_ops;
+
+/* returns the iteration count (LM_FIT's `it`); *chisq is the final chi2 before
+ * the division by normp; *lm_chisq carries the last trial chi2 (H2 state) */
+static int orc_lm_control(const me_config *cf, int regul_any, const orc_lm_ops *o, double *lm_chisq,
+                          double *chisq_out, int32_t *trials)
 {
-    const me_config *cf = &W->c->cfg;
     const double limit = 0.1;
-    orc_synth_w(W, W->P, 1);
-    W->ratio = 1.0;
-    double chisq = orc_loss(W);
-    if (W->c->regul_any) {
-        orc_regul(W, W->P, 0);
-        if (W->penalty > limit * W->lm_chisq) W->ratio = limit * chisq / W->penalty;
+    double chisq = o->start(o->u);
+    double ratio = 1.0;
+    if (regul_any) {
+        const double pen = o->penalty_p(o->u);
+        if (pen > limit * *lm_chisq) ratio = limit * chisq / pen;
     }
     int it;
     int accepted = 0;
     double cn = chisq;
-    double dp = W->damp;
+    const double dp = cf->damp;
+    double damp = dp;
     for (it = 0; it < cf->num_iter; it++) {
         accepted = 0;
-        orc_hessian(W);
-        if (W->c->regul_any) orc_regul(W, W->P, 1);
+        o->normal(o->u, ratio);
         do {
-            cn = orc_trial(W);
+            double pen = 0.0;
+            cn = o->trial(o->u, &pen);
+            (*trials)++;
+            *lm_chisq = cn;
             if (cn < chisq) {
-                if (W->c->regul_any && W->penalty > limit * W->lm_chisq)
-                    W->ratio = limit * chisq / W->penalty;
+                if (regul_any && pen > limit * *lm_chisq) ratio = limit * chisq / pen;
                 accepted = 1;
-                ORC_SWAP(W->P, W->Pt);
-                ORC_SWAP(W->syn, W->fit);
-                orc_synth_w(W, W->P, 1);
-                W->damp /= cf->lambda_accept;
-                if (W->damp < cf->lambda_limits[0]) W->damp = cf->lambda_limits[0];
+                o->accept(o->u);
+                damp /= cf->lambda_accept;
+                if (damp < cf->lambda_limits[0]) damp = cf->lambda_limits[0];
             } else {
-                W->damp *= cf->lambda_reject;
-                if (W->damp > cf->lambda_limits[1]) break;
+                damp *= cf->lambda_reject;
+                if (damp > cf->lambda_limits[1]) break;
             }
         } while (!accepted);
-        W->damp = dp;
+        damp = dp;
         if (accepted) {
             if (((chisq - cn) / chisq) < cf->convg_criteria) {
                 chisq = cn;
@@ -1227,8 +1242,98 @@ static int orc_lm_fit(orc_work *W)
             break;
         }
     }
+    *chisq_out = chisq;
+    return it;
+}
+
+static double orc_op_start(void *u)
+{
+    orc_work *W = (orc_work *)u;
+    orc_synth_w(W, W->P, 1);
+    return orc_loss(W);
+}
+
+static double orc_op_penalty(void *u)
+{
+    orc_work *W = (orc_work *)u;
+    orc_regul(W, W->P, 0);
+    return W->penalty;
+}
+
+static void orc_op_normal(void *u, double ratio)
+{
+    orc_work *W = (orc_work *)u;
+    orc_hessian(W);
+    W->ratio = ratio;
+    if (W->c->regul_any) orc_regul(W, W->P, 1);
+}
+
+static double orc_op_trial(void *u, double *pen)
+{
+    orc_work *W = (orc_work *)u;
+    const double chi = orc_trial(W);
+    *pen = W->penalty;
+    return chi;
+}
+
+static void orc_op_accept(void *u)
+{
+    orc_work *W = (orc_work *)u;
+    ORC_SWAP(W->P, W->Pt);
+    ORC_SWAP(W->syn, W->fit);
+    orc_synth_w(W, W->P, 1);
+}
+
+/* one LM run (Src/LM_FIT.c:531-661); returns the iteration count */
+static int orc_lm_fit(orc_work *W)
+{
+    const orc_lm_ops ops = {W, orc_op_start, orc_op_penalty, orc_op_normal, orc_op_trial, orc_op_accept};
+    double chisq;
+    int32_t tr = 0;
+    const int it = orc_lm_control(&W->c->cfg, W->c->regul_any, &ops, &W->lm_chisq, &chisq, &tr);
+    W->trials += tr;
     W->iters += it;
     W->chisq = chisq / W->normp;
+    return it;
+}
+
+/* replay hook state: chi2 and penalty values handed out in order */
+typedef struct orc_replay {
+    double chi0, pen0;
+    const double *tchi, *tpen;
+    int ntr, k;
+    double *ratios;
+    int nrmax, nr;
+} orc_replay;
+
+static double orc_rp_start(void *u) { return ((orc_replay *)u)->chi0; }
+static double orc_rp_penalty(void *u) { return ((orc_replay *)u)->pen0; }
+static void orc_rp_normal(void *u, double ratio)
+{
+    orc_replay *R = (orc_replay *)u;
+    if (R->nr < R->nrmax) R->ratios[R->nr] = ratio;
+    R->nr++;
+}
+static double orc_rp_trial(void *u, double *pen)
+{
+    orc_replay *R = (orc_replay *)u;
+    const int k = R->k < R->ntr ? R->k : R->ntr - 1;
+    R->k++;
+    *pen = R->tpen ? R->tpen[k] : 0.0;
+    return R->tchi[k];
+}
+static void orc_rp_accept(void *u) { (void)u; }
+
+int orc_lm_replay(const me_config *cfg, int regul_any, double chi0, double pen0, double *lm_chisq,
+                  const double *trial_chi, const double *trial_pen, int ntrial, int *trials, double *chisq,
+                  double *ratios, int nratio_max, int *nratios)
+{
+    orc_replay R = {chi0, pen0, trial_chi, trial_pen, ntrial, 0, ratios, nratio_max, 0};
+    const orc_lm_ops ops = {&R, orc_rp_start, orc_rp_penalty, orc_rp_normal, orc_rp_trial, orc_rp_accept};
+    int32_t tr = 0;
+    const int it = orc_lm_control(cfg, regul_any, &ops, lm_chisq, chisq, &tr);
+    *trials = tr;
+    *nratios = R.nr;
     return it;
 }
 
@@ -1239,17 +1344,22 @@ static void orc_sav(orc_work *W)
     W->chisq_best = W->chisq;
 }
 
-/* Src/LM_FIT.c:157-201 with Philox4x32-10 Box–Muller draws for irun >= 4 */
-static void orc_jump(orc_work *W, int64_t pix, int irun)
+/* Random_Jump (Src/LM_FIT.c:157-201): runs 1-3 restart from the best run
+ * with phi = phi_guess + pi/4 irun; from run 4 on, Gaussian jumps around the
+ * guess (Doppler width around the best, S the best), the eight draws taken in
+ * the reference's order (B, theta, phi, v, Doppler, a, eta0, beta) from two
+ * Philox4x32-10 blocks (counter = pixel key, run, stream) through Box–Muller,
+ * in place of the time-seeded NR gasdev; then bounds_check. */
+static void orc_jump_to(uint64_t seed, int64_t key, int irun, const double *B, const double *G,
+                        const double lim[9][2], double *P)
 {
-    double *P = W->P, *B = W->Pb, *G = W->Pg;
     if (irun < 4) {
         for (int k = 0; k < 9; k++) P[k] = B[k];
         P[2] = G[2] + ORC_PI / 4. * irun;
     } else {
         double n[8];
-        orc_gauss4(W->c->cfg.seed, pix, (uint32_t)irun, ORC_STREAM_JUMP + 0, n);
-        orc_gauss4(W->c->cfg.seed, pix, (uint32_t)irun, ORC_STREAM_JUMP + 1, n + 4);
+        orc_gauss4(seed, key, (uint32_t)irun, ORC_STREAM_JUMP + 0, n);
+        orc_gauss4(seed, key, (uint32_t)irun, ORC_STREAM_JUMP + 1, n + 4);
         P[0] = G[0] * (1 + 0.6 * n[0]);
         P[1] = G[1] + ORC_PI / 8. * n[1];
         P[2] = G[2] + ORC_PI / 4. * (irun + 0.3 * n[2]);
@@ -1260,7 +1370,23 @@ static void orc_jump(orc_work *W, int64_t pix, int irun)
         P[7] = B[7];
         P[8] = G[8] + 0.1 * n[7];
     }
-    orc_bounds_check(P, (const double(*)[2])W->lim);
+    orc_bounds_check(P, lim);
+}
+
+static void orc_jump(orc_work *W, int64_t key, int irun)
+{
+    orc_jump_to(W->c->cfg.seed, key, irun, W->Pb, W->Pg, (const double(*)[2])W->lim, W->P);
+}
+
+void orc_jump_probe(uint64_t seed, int64_t key, int irun, const double *best, const double *guess,
+                    const double *lim, double *par)
+{
+    orc_jump_to(seed, key, irun, best, guess, (const double(*)[2])lim, par);
+}
+
+void orc_gauss4_probe(uint64_t seed, int64_t key, uint32_t c2, uint32_t stream, double *g)
+{
+    orc_gauss4(seed, key, c2, stream, g);
 }
 
 /* save/break decision after run irun (Src/LM_FIT.c:713-724): a run under the
@@ -1282,12 +1408,19 @@ static void orc_policy(int irun, int nruns, double crit, double chisq, double be
     if (irun >= nruns * 2 && best < crit * 2) *brk = 1;
 }
 
-/* Src/LM_FIT.c:665-742; returns the run counter (0 = not inverted) */
-static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init, int32_t *runs)
+/* Src/LM_FIT.c:665-742; returns the run counter (0 = not inverted).
+ * learned != 0: init holds the learned initialiser's prediction; one LM run
+ * starts from it and the pixel is done if its chi2/normp <= chisq_criteria;
+ * otherwise the runs above start from the heuristic guess and the better of
+ * the learned run and their result is kept (*fallback = 1).  key is the
+ * pixel's Philox counter for the restart jumps. */
+static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init, int learned, int64_t key,
+                         int32_t *runs, int *fallback)
 {
     const me_config *cf = &W->c->cfg;
     double lim7[2], step7, norm0;
     *runs = 0;
+    *fallback = 0;
     if (orc_init_guess(W->c, W->prof, pix, n, W->Pg, lim7, &step7, &norm0, &W->normp)) {
         W->chisq_best = -1.;
         W->normp = 1.;
@@ -1297,16 +1430,35 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
     W->lim[7][0] = lim7[0];
     W->lim[7][1] = lim7[1];
     W->step[7] = step7;
-    if (init)
+    if (init && !learned)
         for (int k = 0; k < 9; k++) W->Pg[k] = init[k];
     orc_bounds_check(W->Pg, (const double(*)[2])W->lim);
     for (int k = 0; k < 9; k++) W->P[k] = W->Pg[k];
 
     for (int s = 0; s < 4; s++) W->iv[s] = W->c->wsq[s] * (1. / norm0);
 
+    double PL[9], chiL = 0.0;
+    if (learned) {
+        for (int k = 0; k < 9; k++) W->P[k] = init[k];
+        orc_bounds_check(W->P, (const double(*)[2])W->lim);
+        orc_lm_fit(W);
+        (*runs)++;
+        if (W->chisq <= cf->chisq_criteria || cf->num_run == 0) {
+            orc_sav(W);
+            goto done;
+        }
+        *fallback = 1;
+        for (int k = 0; k < 9; k++) {
+            PL[k] = W->P[k];
+            W->P[k] = W->Pg[k];
+        }
+        chiL = W->chisq;
+        memcpy(W->fitl, W->fit, sizeof(double) * 4 * W->nl);
+    }
+
     int irun;
     for (irun = 0; irun < cf->num_run * 4; irun++) {
-        if (irun > 0) orc_jump(W, pix, irun);
+        if (irun > 0) orc_jump(W, key, irun);
         orc_lm_fit(W);
         (*runs)++;
         int save, brk;
@@ -1314,12 +1466,18 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
         if (save) orc_sav(W);
         if (brk) break;
     }
+    if (*fallback && chiL < W->chisq_best) {
+        for (int k = 0; k < 9; k++) W->Pb[k] = PL[k];
+        W->chisq_best = chiL;
+        memcpy(W->fitb, W->fitl, sizeof(double) * 4 * W->nl);
+    }
+done:
     if (W->Pb[1] > ORC_PI) W->Pb[1] -= ORC_PI;
     if (cf->hmi_ref) {
         W->Pb[2] += 0.5 * ORC_PI;
         if (W->Pb[2] > ORC_PI) W->Pb[2] -= ORC_PI;
     }
-    return irun;
+    return *runs;
 }
 
 int orc_run_policy(int nruns, double criteria, const double *chisq_seq, int nseq, int *best_run)
@@ -1337,6 +1495,9 @@ int orc_run_policy(int nruns, double criteria, const double *chisq_seq, int nseq
     }
     return nexec;
 }
+
+/* per-thread buffers: syn, fit, fitb, fitl [4][nl] each + Jacobian [4][9][nl] */
+#define ORC_WORK_DOUBLES(nl) (52 * (nl) + 8)
 
 static void orc_work_init(orc_work *W, const orc_ctx *c, double *mem)
 {
@@ -1356,14 +1517,15 @@ static void orc_work_init(orc_work *W, const orc_ctx *c, double *mem)
     W->syn = mem;
     W->fit = mem + 4 * nl;
     W->fitb = mem + 8 * nl;
-    W->jac = mem + 12 * nl;
+    W->fitl = mem + 12 * nl;
+    W->jac = mem + 16 * nl;
     W->damp = c->cfg.damp;
     W->lm_chisq = 0.0;
 }
 
 /* one pixel of INVERSION_MULTI (Src/LM_FIT.c:774-808) */
-static void orc_pixel(orc_work *W, const double *prof, int64_t p, int64_t n, double *res,
-                      double *fit, const double *init, int h2_mode, orc_pixstats *ps)
+static void orc_pixel(orc_work *W, const double *prof, int64_t p, int64_t n, double *res, double *fit,
+                      const double *init, int learned, int64_t key, int h2_mode, orc_pixstats *ps)
 {
     const int nl = W->nl;
     W->prof = (double *)prof;
@@ -1373,9 +1535,10 @@ static void orc_pixel(orc_work *W, const double *prof, int64_t p, int64_t n, dou
     mean /= nl;
     int valid = !(isnan(mean) || mean < W->c->cfg.icriteria);
     int32_t runs = 0;
+    int fallback = 0;
     if (valid) {
         if (h2_mode == ORC_H2_PER_PIXEL) W->lm_chisq = INFINITY;
-        if (!orc_inversion(W, p, n, init, &runs)) valid = 0;
+        if (!orc_inversion(W, p, n, init, learned, key, &runs, &fallback)) valid = 0;
     }
     if (valid) {
         for (int k = 0; k < 9; k++) res[k] = W->Pb[k];
@@ -1389,29 +1552,32 @@ static void orc_pixel(orc_work *W, const double *prof, int64_t p, int64_t n, dou
         ps->iters = W->iters;
         ps->runs = runs;
         ps->trials = W->trials;
-        ps->status = (valid ? 0 : ME_PIX_INVALID) | (W->svdfail ? ME_PIX_SVD_FAIL : 0);
+        ps->status = (valid ? 0 : ME_PIX_INVALID) | (W->svdfail ? ME_PIX_SVD_FAIL : 0) |
+                     (valid && fallback ? ME_PIX_FALLBACK : 0);
     }
 }
 
-void orc_invert(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
-                const double *init, int h2_mode, int nthreads, orc_pixstats *pst, int64_t *ev)
+void orc_invert_ex(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
+                   const double *init, int learned, int64_t pix_base, int h2_mode, int nthreads,
+                   orc_pixstats *pst, int64_t *ev)
 {
     const int nl = c->nl;
     int64_t evt[13] = {0};
     if (nthreads < 1) nthreads = 1;
+    if (learned && !init) learned = 0;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nthreads)
 #endif
     {
         orc_work W;
-        double *mem = (double *)malloc(sizeof(double) * (48 * nl + 8));
+        double *mem = (double *)malloc(sizeof(double) * (ORC_WORK_DOUBLES(nl)));
         orc_work_init(&W, c, mem);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 20)
 #endif
         for (int64_t p = 0; p < n; p++) {
             orc_pixel(&W, prof + 4 * nl * p, p, n, res + 10 * p, fit ? fit + 4 * nl * p : NULL,
-                      init ? init + 9 * p : NULL, h2_mode, pst ? pst + p : NULL);
+                      init ? init + 9 * p : NULL, learned, pix_base + p, h2_mode, pst ? pst + p : NULL);
         }
 #ifdef _OPENMP
 #pragma omp critical
@@ -1421,6 +1587,12 @@ void orc_invert(const orc_ctx *c, const double *prof, int64_t n, double *res, do
     }
     if (ev)
         for (int k = 0; k < 13; k++) ev[k] = evt[k];
+}
+
+void orc_invert(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
+                const double *init, int h2_mode, int nthreads, orc_pixstats *pst, int64_t *ev)
+{
+    orc_invert_ex(c, prof, n, res, fit, init, 0, 0, h2_mode, nthreads, pst, ev);
 }
 
 /* ======================================================================== */
@@ -1467,7 +1639,7 @@ void orc_normal_eq(const orc_ctx *c, const double *par, const double *prof, doub
                    double *H, double *g, double *chi2)
 {
     orc_work W;
-    double *mem = (double *)malloc(sizeof(double) * (48 * c->nl + 8));
+    double *mem = (double *)malloc(sizeof(double) * (ORC_WORK_DOUBLES(c->nl)));
     orc_work_init(&W, c, mem);
     W.step[7] = step7;
     W.prof = (double *)prof;
@@ -1480,11 +1652,28 @@ void orc_normal_eq(const orc_ctx *c, const double *par, const double *prof, doub
     free(mem);
 }
 
+/* Get_Regul with the matrix flag on an npar x npar normal matrix H and
+ * gradient g (packed in parameter order); returns the penalty */
+double orc_regul_probe(const orc_ctx *c, const double *par, double ratio, double *H, double *g)
+{
+    orc_work W;
+    double *mem = (double *)malloc(sizeof(double) * (ORC_WORK_DOUBLES(c->nl)));
+    orc_work_init(&W, c, mem);
+    memcpy(W.H, H, sizeof(double) * W.np * W.np);
+    memcpy(W.g, g, sizeof(double) * W.np);
+    W.ratio = ratio;
+    orc_regul(&W, par, 1);
+    memcpy(H, W.H, sizeof(double) * W.np * W.np);
+    memcpy(g, W.g, sizeof(double) * W.np);
+    free(mem);
+    return W.penalty;
+}
+
 /* truncated SVD solve of SVD_solve_mem; H is replaced by U (H1) */
 void orc_solve(const orc_ctx *c, double *H, const double *g, double *sol)
 {
     orc_work W;
-    double *mem = (double *)malloc(sizeof(double) * (48 * c->nl + 8));
+    double *mem = (double *)malloc(sizeof(double) * (ORC_WORK_DOUBLES(c->nl)));
     orc_work_init(&W, c, mem);
     memcpy(W.H, H, sizeof(double) * W.np * W.np);
     memcpy(W.g, g, sizeof(double) * W.np);

@@ -27,6 +27,21 @@
  *                     branch thresholds, H4 and H7 on constructed grids
  *   run policy H3/H9  run counts and save order on constructed chi2 sequences
  *                     through the orc_run_policy hook
+ *   LM loop control   trial counts under the damping bound, the damping reset
+ *                     per iteration, the accept and convergence tests, the
+ *                     num_iter cap and the regularisation ratio updates, on
+ *                     constructed chi2/penalty sequences through orc_lm_replay,
+ *                     which runs the same control function as the oracle's LM
+ *   Get_Regul         the regularised normal equations against the
+ *                     Gauss-Newton equations of the augmented residual vector
+ *                     (numpy), with fixed parameters shifting the packed index
+ *   Random_Jump       runs 1-3 by hand; runs >= 4 by recovering the eight
+ *                     draws from the jumped parameters (which draw moves which
+ *                     parameter, guess or best as the base), Box-Muller draws
+ *                     tested for N(0,1) (Kolmogorov-Smirnov) and independence
+ *   learned policy    constructed cases: a prediction at the truth converges in
+ *                     one run; a bad prediction falls back to the heuristic
+ *                     runs and keeps the better record
  * Parity unpinned: the end-to-end INVERSION records themselves — the reference's
  * ME_SOLVER.c/LM_FIT.c include fitsio.h, which the image lacks, so they cannot
  * be built here; each stage is pinned as listed instead.
@@ -104,6 +119,16 @@ typedef struct orc_pixstats {
     int32_t iters, runs, trials, status;
 } orc_pixstats;
 
+/* orc_invert with the learned policy and the restart key:
+ *   learned   != 0: init holds the learned initialiser's predictions; one LM
+ *             run from each, heuristic runs (Init_Guess, H3/H9) only when its
+ *             chi2/normp exceeds chisq_criteria, the better record kept
+ *             (status ME_PIX_FALLBACK)
+ *   pix_base  global index of pixel 0: the Philox counter of restart jumps */
+void orc_invert_ex(const orc_ctx *c, const double *prof, int64_t n, double *res, double *fit,
+                   const double *init, int learned, int64_t pix_base, int h2_mode, int nthreads,
+                   orc_pixstats *pst, int64_t *ev);
+
 /* Invert n pixels (double profiles).  init may be NULL.  pst may be NULL.
  * ev (may be NULL) receives event counts: [0]=jac [1]=trial [2]=hess [3]=svd
  * [4]=bk [5..12]=Faddeeva branches.  nthreads > 1 splits pixels into chunks of
@@ -126,6 +151,28 @@ void orc_gen_map(const orc_ctx *c, uint64_t seed, const double *ranges, double f
  * (Src/LM_FIT.c:701-724) on a given per-run chi2 sequence.  Returns the
  * number of runs executed; *best_run is the index whose record is kept. */
 int orc_run_policy(int nruns, double criteria, const double *chisq_seq, int nseq, int *best_run);
+
+/* LM-control hook for tests: runs the oracle's LM loop control (the function
+ * orc_invert's LM uses) with the start chi2 chi0 and penalty pen0, and the
+ * k-th trial returning trial_chi[k] / trial_pen[k] (the last entry repeats).
+ * *lm_chisq is the H2 state in and out.  Returns the iteration count; writes
+ * the trial count, the final chi2 and the regularisation ratio handed to each
+ * normal-matrix build (up to nratio_max, *nratios = number of builds). */
+int orc_lm_replay(const me_config *cfg, int regul_any, double chi0, double pen0, double *lm_chisq,
+                  const double *trial_chi, const double *trial_pen, int ntrial, int *trials, double *chisq,
+                  double *ratios, int nratio_max, int *nratios);
+
+/* Get_Regul with the matrix flag on a packed npar x npar H and g[npar]
+ * (in place); returns the penalty */
+double orc_regul_probe(const orc_ctx *c, const double *par, double ratio, double *H, double *g);
+
+/* Random_Jump for run irun of a pixel with restart key `key`: best, guess,
+ * lim [9][2] in, par out */
+void orc_jump_probe(uint64_t seed, int64_t key, int irun, const double *best, const double *guess,
+                    const double *lim, double *par);
+
+/* four N(0,1) draws (two Box-Muller pairs) of Philox block (key, c2, stream) */
+void orc_gauss4_probe(uint64_t seed, int64_t key, uint32_t c2, uint32_t stream, double *g);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,15 @@ def lib():
         L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_run_policy.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _ip]
         L.orc_run_policy.restype = C.c_int
+        L.orc_invert_ex.argtypes = [C.c_void_p, _dp, C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int64, C.c_int,
+                                    C.c_int, C.POINTER(PixStats), C.POINTER(C.c_int64)]
+        L.orc_lm_replay.argtypes = [C.POINTER(MeConfig), C.c_int, C.c_double, C.c_double, _dp, _dp, _dp,
+                                    C.c_int, _ip, _dp, _dp, C.c_int, _ip]
+        L.orc_lm_replay.restype = C.c_int
+        L.orc_regul_probe.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
+        L.orc_regul_probe.restype = C.c_double
+        L.orc_jump_probe.argtypes = [C.c_uint64, C.c_int64, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_gauss4_probe.argtypes = [C.c_uint64, C.c_int64, C.c_uint32, C.c_uint32, _dp]
         _lib = L
     return _lib
 
@@ -174,6 +183,39 @@ def run_policy(nruns, criteria, seq):
     return n, best.value
 
 
+def lm_replay(cfg, chi0, trial_chi, trial_pen=None, pen0=0.0, lm_chisq=np.inf, regul=False, nratio=256):
+    """Run the oracle's LM-loop control on a constructed sequence: start chi2
+    chi0 (penalty pen0), trial k returns trial_chi[k] (penalty trial_pen[k])."""
+    tc = np.ascontiguousarray(trial_chi, dtype=np.float64)
+    tp = None if trial_pen is None else np.ascontiguousarray(trial_pen, dtype=np.float64)
+    lm = C.c_double(lm_chisq)
+    ntr, nr = C.c_int(), C.c_int()
+    chi = C.c_double()
+    ratios = np.zeros(nratio)
+    it = lib().orc_lm_replay(C.byref(cfg), int(bool(regul)), chi0, pen0, C.byref(lm), _ptr(tc, _dp), _ptr(tp, _dp),
+                             tc.size, C.byref(ntr), C.byref(chi), _ptr(ratios, _dp), nratio, C.byref(nr))
+    return dict(iters=it, trials=ntr.value, chisq=chi.value, lm_chisq=lm.value,
+                ratios=ratios[:min(nr.value, nratio)].copy())
+
+
+def jump(seed, key, irun, best, guess, limits):
+    b = np.ascontiguousarray(best, dtype=np.float64)
+    g = np.ascontiguousarray(guess, dtype=np.float64)
+    lim = np.ascontiguousarray(limits, dtype=np.float64).reshape(9, 2)
+    p = np.empty(9)
+    lib().orc_jump_probe(seed, key, irun, _ptr(b, _dp), _ptr(g, _dp), _ptr(lim, _dp), _ptr(p, _dp))
+    return p
+
+
+def gauss4(seed, key, c2, stream):
+    g = np.empty(4)
+    lib().orc_gauss4_probe(seed, key, c2, stream, _ptr(g, _dp))
+    return g
+
+
+STREAM_JUMP = 0x4A554D00
+
+
 # ---------------------------------------------------------------- context
 class Oracle:
     """One wavelength grid + configuration (INV_INIT derived constants)."""
@@ -253,6 +295,15 @@ class Oracle:
         lib().orc_solve(self.h, _ptr(Hb, _dp), _ptr(gb, _dp), _ptr(sol, _dp))
         return sol[:n].copy(), Hb[:n * n].reshape(n, n).copy()
 
+    def regul(self, par, ratio, H, g):
+        """Get_Regul with the matrix flag applied to a packed H, g (copies)."""
+        n = self.npar
+        Hb = np.ascontiguousarray(H, dtype=np.float64).reshape(n * n).copy()
+        gb = np.ascontiguousarray(g, dtype=np.float64).reshape(n).copy()
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        pen = lib().orc_regul_probe(self.h, _ptr(par, _dp), ratio, _ptr(Hb, _dp), _ptr(gb, _dp))
+        return pen, Hb.reshape(n, n), gb
+
     def init_guess(self, prof_batch, pix=0):
         prof_batch = np.ascontiguousarray(prof_batch, dtype=np.float64).reshape(-1, 4, self.nl)
         g = np.zeros(9)
@@ -263,16 +314,22 @@ class Oracle:
                                  _ptr(l7, _dp), C.byref(s7), C.byref(n0), C.byref(npn))
         return dict(fail=r, guess=g, limit7=l7, step7=s7.value, norm0=n0.value, normp=npn.value)
 
-    def invert(self, prof, init=None, fit=False, h2=ORC_H2_PER_PIXEL, nthreads=1):
+    def invert(self, prof, init=None, fit=False, h2=ORC_H2_PER_PIXEL, nthreads=1, learned=False, pix_base=0):
+        """learned: init holds the learned initialiser's predictions and the
+        learned policy runs (one LM run, heuristic fallback above
+        chisq_criteria); pix_base: global index of pixel 0 (restart key)."""
         prof = np.ascontiguousarray(prof, dtype=np.float64).reshape(-1, 4, self.nl)
         n = prof.shape[0]
         res = np.empty((n, 10))
         F = np.empty((n, 4, self.nl)) if fit else None
         ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64).reshape(n, 9)
+        if learned and ini is None:
+            raise ValueError("learned policy needs the predictions in init")
         ps = (PixStats * max(n, 1))()
         ev = np.zeros(13, dtype=np.int64)
-        lib().orc_invert(self.h, _ptr(prof, _dp), n, _ptr(res, _dp), _ptr(F, _dp), _ptr(ini, _dp),
-                         h2, nthreads, ps, ev.ctypes.data_as(C.POINTER(C.c_int64)))
+        lib().orc_invert_ex(self.h, _ptr(prof, _dp), n, _ptr(res, _dp), _ptr(F, _dp), _ptr(ini, _dp),
+                            int(bool(learned)), int(pix_base), h2, nthreads, ps,
+                            ev.ctypes.data_as(C.POINTER(C.c_int64)))
         st = np.array([(ps[i].iters, ps[i].runs, ps[i].trials, ps[i].status) for i in range(n)],
                       dtype=np.int32).reshape(n, 4)
         out = dict(res=res, iters=st[:, 0], runs=st[:, 1], trials=st[:, 2], status=st[:, 3], events=ev)

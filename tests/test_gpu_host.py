@@ -100,6 +100,19 @@ def test_host_interrupted_run_resumes_from_the_cache(tmp_path):
     assert flags[:-1].all()
 
 
+def test_host_resume_with_restarts_equals_an_uninterrupted_run(tmp_path):
+    """num_run = 2 (Gaussian jumps at irun >= 4): the host keys each pixel's
+    restart draws by its index in the box (me_ctx_set_pixel_base), so a run
+    stopped after 5 chunks and resumed with other batch boundaries ends with
+    the records of one uninterrupted batch."""
+    w, ref = _setup(tmp_path, nprofiles=50, Cache_inv="YES", num_run=2)
+    _run(tmp_path, "--max-chunks", 5, "--batch", 100)
+    out = _run(tmp_path, "--batch", 150)
+    assert "(resumed)" in out
+    _, res = _read_fits(tmp_path / "res.fits")
+    np.testing.assert_array_equal(res.reshape(-1, 10), ref["res"])
+
+
 def test_batch_driver_inverts_every_cube_with_one_context(tmp_path, monkeypatch):
     """meinv.batch (Tools/RunMEINV.py's command line): two cubes in a
     directory, one host process, res/<stem>_res.fits each equal to the library

@@ -1,0 +1,82 @@
+"""Learned mode judged by the FP64 oracle: the network's predictions are
+copied off the device and handed to the oracle's learned policy (one LM run
+from the prediction; heuristic runs only above chisq_criteria; the better
+record kept), on the same hmi4k pixels the engine inverts."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import DEG, device_tensor, gpu_inversion, within_parity, within_physical
+from meinv import ME_FP32, ME_INIT_LEARNED
+from meinv.workloads import get
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+
+
+@pytest.fixture(scope="module")
+def learned_case():
+    import torch
+    w = get("hmi4k")
+    ranges = w.ranges()
+    with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED) as inv:
+        inv.mlp_train(w.seed + 100, ranges, 262144, batch=2048, epochs=4, lr=2e-3, ff_frac=w.ff_frac,
+                      ff_bmax=w.ff_bmax, sigma=w.sigma)
+        params = inv.mlp_params()
+        d_prof = device_tensor((N, 4, w.nl), torch.float32)
+        inv.gen_map_device(w.seed, ranges, N, d_prof.data_ptr(), None, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax,
+                           sigma=w.sigma)
+        d_init = device_tensor((N, 9), torch.float64)
+        inv.mlp_predict_device(d_prof.data_ptr(), N, d_init.data_ptr())
+        torch.cuda.synchronize()
+        prof = d_prof.cpu().numpy()
+        pred = d_init.cpu().numpy()
+        d_res = device_tensor((N, 10), torch.float64)
+        inv.invert_device(d_prof.data_ptr(), N, d_res.data_ptr())
+        res32_net = d_res.cpu().numpy()
+    return w, params, prof, pred, res32_net
+
+
+def test_learned_fp64_engine_matches_the_oracle(learned_case):
+    """FP64 engine in learned mode seeded by the predictions vs the oracle's
+    learned policy seeded by the same values: agreement at the chaos floor."""
+    w, params, prof, pred, _ = learned_case
+    prof64 = prof.astype(np.float64)
+    orc = O.Oracle(w.config(init_mode=ME_INIT_LEARNED), w.wavelengths())
+    ref = orc.invert(prof64, init=pred, learned=True, nthreads=8)
+    rng = np.random.default_rng(0)
+    self_ = orc.invert(prof64 * (1 + 1e-15 * rng.standard_normal(prof64.shape)), init=pred, learned=True, nthreads=8)
+    floor = within_parity(self_["res"], ref["res"]).mean()
+    with gpu_inversion("hmi4k", init_mode=ME_INIT_LEARNED) as inv:
+        out = inv.invert(prof64, init=pred, per_pixel=True)
+    ok = within_parity(out["res"], ref["res"])
+    fb_dev = (out["stats"].per_pixel["status"] & 4) != 0
+    fb_orc = (ref["status"] & 4) != 0
+    conv = np.mean(out["res"][:, 9] <= w.chisq_criteria)
+    conv_o = np.mean(ref["res"][:, 9] <= w.chisq_criteria)
+    print(f"learned FP64 engine vs oracle: {ok.mean():.4f} within tolerance (oracle self {floor:.4f}); fallback "
+          f"{fb_dev.mean():.4f} / {fb_orc.mean():.4f}; converged {conv:.4f} / {conv_o:.4f}")
+    assert np.mean(fb_dev == fb_orc) >= 0.99
+    assert ok.mean() >= floor - 0.03
+
+
+def test_learned_fp32_error_against_the_oracle(learned_case):
+    """The headline form (FP32 synthesis, network prediction) against the FP64
+    oracle's learned policy from the same predictions: per-parameter error and
+    the fraction of pixels within 0.1 G / 0.01 deg / 1 m/s."""
+    w, params, prof, pred, res32 = learned_case
+    orc = O.Oracle(w.config(init_mode=ME_INIT_LEARNED), w.wavelengths())
+    ref = orc.invert(prof.astype(np.float64), init=pred, learned=True, nthreads=8)["res"]
+    conv_a = np.mean(res32[:, 9] <= w.chisq_criteria)
+    conv_b = np.mean(ref[:, 9] <= w.chisq_criteria)
+    both = (res32[:, 9] <= w.chisq_criteria) & (ref[:, 9] <= w.chisq_criteria)
+    phys_all = within_physical(res32, ref).mean()
+    phys = within_physical(res32[both], ref[both]).mean()
+    d = np.abs(res32[both] - ref[both])
+    print(f"learned FP32 engine vs FP64 oracle: converged {conv_a:.4f} / {conv_b:.4f}; within 0.1 G / 0.01 deg / "
+          f"1 m/s: {phys_all:.4f} of all pixels, {phys:.4f} of pixels converged in both; median |dB| "
+          f"{np.median(d[:, 0]):.3g} G, |dtheta| {np.median(d[:, 1]) * DEG:.3g} deg, |dv| "
+          f"{np.median(d[:, 3]) * 1e3:.3g} m/s")
+    assert abs(conv_a - conv_b) <= 0.02
+    assert phys >= 0.6

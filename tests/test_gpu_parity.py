@@ -4,7 +4,7 @@ import pytest
 
 import oracle as O
 from gpu_util import DEG, chaos_floor, device_tensor, gpu_inversion, oracle_map, within_parity, within_physical
-from meinv import ME_FP32
+from meinv import ME_FP32, Inversion
 from meinv.workloads import get
 
 pytestmark = pytest.mark.gpu
@@ -283,3 +283,62 @@ def test_split_jacobian_is_independent_of_the_launch_shape():
     assert np.mean(outs[0][:, 9] > 0) > 0.9
     np.testing.assert_array_equal(outs[1], outs[0])
     np.testing.assert_array_equal(outs[2], outs[0])
+
+
+@pytest.mark.parametrize("name,n", [("hinode", 512), ("hmi4k", 2048)])
+def test_inversion_fp32_plain_error_against_fp64_oracle(name, n):
+    """FP32 plain inversion (hinode: nl = 112, the Jacobian pass split over
+    two wavelength halves; hmi4k: the headline precision, nl = 6) against the
+    FP64 oracle from the same heuristic start: converged fraction within 2
+    points; within-tolerance fraction and median differences reported."""
+    w, orc, prof, truth = oracle_map(name, n)
+    ref = orc.invert(prof.astype(np.float64), nthreads=8)
+    with gpu_inversion(name, precision=ME_FP32) as inv:
+        out = inv.invert(prof)
+    a, b = out["res"], ref["res"]
+    conv_a = np.mean(a[:, 9] <= w.chisq_criteria)
+    conv_b = np.mean(b[:, 9] <= w.chisq_criteria)
+    d = np.abs(a - b)
+    phys = within_physical(a, b).mean()
+    print(f"{name} FP32 plain vs FP64 oracle: {phys:.4f} of pixels within 0.1 G / 0.01 deg / 1 m/s; converged "
+          f"{conv_a:.4f} (oracle {conv_b:.4f}); median |dB| {np.median(d[:, 0]):.3g} G, |dtheta| "
+          f"{np.median(d[:, 1]) * DEG:.3g} deg, |dv| {np.median(d[:, 3]) * 1e3:.3g} m/s")
+    assert abs(conv_a - conv_b) <= 0.02
+    assert phys >= 0.6
+
+
+def test_synthesis_fp64_special_lines(golden):
+    """Faddeeva's special lines on the device (Src/FADDEEVA.c:461-468): a = 0
+    puts every call on y = 0 (exp(-nu^2), Dawson); B = 0 and v_los = 0 on the
+    NIRIS grid, whose sample 13 is lambda0 exactly, put nu = 0 there (erfcx)."""
+    lam = np.load(golden / "niris_wavelength.npy")
+    w = get("niris64")
+    orc = O.Oracle(w.config(), lam)
+    par = _random_params(w, 64, seed=5)
+    par[:32, 5] = 0.0                       # y = 0 everywhere
+    par[32:, 0] = 0.0                       # B = 0 ...
+    par[32:, 3] = 0.0                       # ... and v_los = 0: nu = 0 at lambda[13]
+    _, c0 = orc.synth_counted(par[0])
+    _, c1 = orc.synth_counted(par[40])
+    assert c0[6] + c0[7] == 3 * lam.size and c0[6] > 0 and c1[7] == 3
+    with Inversion(w.config(), lam) as inv:
+        syn, J = inv.synth(par, jac=True)
+    ref, Jr = orc.synth(par, jac=True)
+    assert np.max(np.abs(syn - ref) / par[:, 7][:, None, None]) <= 1e-12
+    jscale = np.abs(Jr).max(axis=(1, 3), keepdims=True) + 1e-300
+    assert np.max(np.abs(J - Jr) / jscale) <= 1e-9
+
+
+def test_context_constants_match_the_oracle(golden):
+    """me_ctx_info (INV_INIT's derived constants, Src/INV_INIT.c:121-177)
+    against the oracle: the NIRIS grid (H4: ic = 12) and the workload grids."""
+    grids = [("niris64", np.load(golden / "niris_wavelength.npy"))]
+    grids += [(name, get(name).wavelengths()) for name in ("niris64", "hinode", "hmi4k")]
+    for name, lam in grids:
+        cfg = get(name).config()
+        ref = O.Oracle(cfg, lam).info()
+        with Inversion(cfg, lam) as inv:
+            got = inv.info()
+        assert (got["ic"], got["i0"], got["i1"], got["ncut"]) == (ref["ic"], ref["i0"], ref["i1"], ref["ncut"]), name
+        assert tuple(got["vlos"]) == tuple(ref["vlos"]), name
+    assert O.Oracle(get("niris64").config(), grids[0][1]).info()["ic"] == 12

@@ -144,3 +144,65 @@ def test_invalid_pixels_give_zero_records():
     assert np.all(out["res"][1] == 0) and np.all(out["res"][2] == 0)
     assert np.all(out["res"][[0, 3], 9] > 0)
     assert list(out["status"][[1, 2]]) == [1, 1]
+
+
+def test_init_guess_transverse_field_and_the_b_table():
+    """B_pos = min(100 + LCoeffi * Lmax * factor, 2200) with Lmax the largest
+    sqrt(Q^2 + U^2) over i0..i1 in percent of mean(I) (Src/ME_SOLVER.c:525-
+    545); B = hypot(B_los, B_pos), theta = acos(B_los / B); off the NIRIS line
+    the starting Doppler width, damping, eta0 and beta come from the B > 1100
+    or B <= 1100 row of the table (:631-660).  H8: sqrt(Q^2+U^2) is compared
+    with the threshold 5 in absolute units, so phi stays pi/2."""
+    w = get("hmi4k")
+    orc = O.Oracle(w.config(), w.wavelengths())
+    lam = w.wavelengths()
+    info = orc.info()
+    win = slice(info["i0"], info["i1"] + 1)
+    I = 1.0 - 0.5 * np.exp(-((lam - w.lambda0 - 0.01) / 0.08) ** 2)
+    shape = np.exp(-((lam - w.lambda0) / 0.08) ** 2)
+    for amp, row in ((0.06, (20.0, 0.5, 20.0, 0.3)), (0.01, (30.0, 0.5, 5.0, 0.15))):
+        Q, U = 0.6 * amp * shape, -0.8 * amp * shape
+        prof = np.stack([I, Q, U, 0 * I])[None]
+        g = orc.init_guess(prof)["guess"]
+        Imean = I.mean()
+        Lmax = 100 * np.max(np.hypot(Q[win], U[win])) / Imean
+        Bpos = min(100 + 250 * Lmax * 1.0, 2200)
+        np.testing.assert_allclose(g[0], Bpos, rtol=1e-14)           # B_los = 0
+        np.testing.assert_allclose(g[1], np.pi / 2, rtol=1e-15)
+        np.testing.assert_allclose(g[2], np.pi / 2, rtol=1e-15)      # H8
+        assert (g[4], g[5], g[6], g[8]) == row, (amp, g)
+        assert (Bpos > 1100) == (row[0] == 20.0)
+    # the cap: a very strong Q/U signal gives B_pos = 2200
+    prof = np.stack([I, 0.5 * shape, 0 * I, 0 * I])[None]
+    assert orc.init_guess(prof)["guess"][0] == 2200.0
+
+
+def test_init_guess_weak_line_doubles_the_field_coefficients():
+    """A line shallower than 10% (Imin > 0.9 Imax) doubles the V and L
+    coefficients (factor 2, Src/ME_SOLVER.c:523-531), and v_los cannot come
+    from the I minimum (the five-point check needs Imin < 0.9 Imax): below the
+    V threshold it is the I-weighted centroid over ic +- 7."""
+    w, orc = _grid()
+    lam = w.wavelengths()
+    info = orc.info()
+    win = slice(info["i0"], info["i1"] + 1)
+    I = 1.0 - 0.05 * np.exp(-((lam - w.lambda0 - 0.05) / 0.25) ** 2)
+    V = 0.004 * (lam - w.lambda0) / 0.25 * np.exp(-((lam - w.lambda0) / 0.25) ** 2)
+    Q = 0.002 * np.exp(-((lam - w.lambda0) / 0.25) ** 2)
+    prof = np.stack([I, Q, 0 * I, V])[None]
+    g = orc.init_guess(prof)["guess"]
+    Imean = I.mean()
+    LV = 0.5 * (100 * V[win].max() / Imean - 100 * V[win].min() / Imean)
+    assert 0.1 < LV < 1.5
+    iVmin = info["i0"] + int(np.argmin(V[win]))
+    iVmax = info["i0"] + int(np.argmax(V[win]))
+    Blos = 20 + 100 * LV * 2.0
+    Blos = -Blos if iVmin < iVmax else Blos
+    Bpos = min(100 + 250 * (100 * np.max(np.abs(Q[win])) / Imean) * 2.0, 2200)
+    B = np.hypot(Blos, Bpos)
+    np.testing.assert_allclose(g[0], B, rtol=1e-13)
+    np.testing.assert_allclose(g[1], np.arccos(Blos / B), rtol=1e-13)
+    a, b = max(info["ic"] - 7, 0), min(info["ic"] + 7, w.nl - 1)
+    d = I.max() - I[a:b + 1]
+    cen = np.sum(lam[a:b + 1] * d) / np.sum(d) - w.lambda0
+    np.testing.assert_allclose(g[3], cen / w.lambda0 * C_LIGHT / 1e3, rtol=1e-11)
