@@ -175,6 +175,25 @@ me_status me_invert_device(me_ctx *ctx, const void *d_prof, int prof_is_f64, int
                            double *d_res, double *d_fit, const double *d_init, me_stats *stats);
 
 /*
+ * me_invert_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The kernel's per-lane scratch is a static device array of the library
+ * (9 + 2*81 doubles for each of at most 65536 resident lanes, 89.6 MB per GPU,
+ * held until the process ends) instead of the context's buffer; the work and
+ * event counters are the context's and are cleared by a fill kernel.  Same
+ * results as me_invert_device.  Limits: no best-fit profiles; learned mode
+ * needs d_init (predictions from me_mlp_predict_device, which may allocate
+ * on its first call); stats may be NULL or hold NULL per-pixel arrays
+ * (ME_EINVAL otherwise); an explicit launch shape (block_size /
+ * waves_per_cu) needing more than 65536 lanes is ME_EINVAL, the default one
+ * is capped there.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (from two contexts or two host threads) race.
+ */
+me_status me_invert_device_noalloc(me_ctx *ctx, const void *d_prof, int prof_is_f64, int64_t n,
+                                   double *d_res, const double *d_init, me_stats *stats);
+
+/*
  * Forward synthesis with optional analytic Jacobian.  Replaces
  * Milne_Eddington_Single (Src/ME_SOLVER.c:101-366).
  *   par    [n][9]        host double

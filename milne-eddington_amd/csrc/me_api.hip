@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/meinv.h"
@@ -119,17 +120,45 @@ void persistent_grid(const me_ctx *c, K kernel, int64_t n, int &block, int &grid
     grid = (int)std::max<int64_t>(1, std::min(g, need));
 }
 
+// clears the work counter and the event counters of the no-alloc path (a
+// fill kernel rather than hipMemsetAsync, which graph capture records badly)
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ext_scratch: the no-alloc path's static scratch array (device address);
+// null: the context's own buffer, grown on demand
 template <typename T, typename TIn>
 me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res, double *d_fit,
-                        const double *d_init, int learned, int32_t *d_px, me_stats *st)
+                        const double *d_init, int learned, int32_t *d_px, me_stats *st,
+                        double *ext_scratch = nullptr)
 {
     auto kern = me::invert_kernel<T, TIn>;
     int block, grid;
     persistent_grid(c, kern, n, block, grid);
+    double *scratch = nullptr;
+    if (ext_scratch) {
+        const int64_t cap = me::kNoAllocLanes / block;
+        if (grid > cap) {
+            // an explicit launch shape beyond the static array is an error; the
+            // default shape (all resident waves) is capped at the array's lanes
+            if (c->cfg.waves_per_cu > 0 || c->cfg.block_size > 0) return ME_EINVAL;
+            grid = (int)cap;
+        }
+        scratch = ext_scratch;
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_queue, 1);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_counters, (int)me::CT_N);
+        HIPCHK(hipGetLastError());
+    } else {
+        HIPCHK(c->scratch.ensure(sizeof(double) * me::SL_N * (int64_t)grid * block));
+        scratch = (double *)c->scratch.p;
+        HIPCHK(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned long long), c->stream));
+        HIPCHK(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * me::CT_N, c->stream));
+    }
     const int64_t nslots = (int64_t)grid * block;
-    HIPCHK(c->scratch.ensure(sizeof(double) * me::SL_N * nslots));
-    HIPCHK(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned long long), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * me::CT_N, c->stream));
     if (d_fit) HIPCHK(c->fitpar.ensure(sizeof(double) * 9 * n));
     me::InvArgs A;
     A.prof = d_prof;
@@ -139,7 +168,7 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
     A.fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
     A.init = d_init;
     A.learned = learned;
-    A.scratch = (double *)c->scratch.p;
+    A.scratch = scratch;
     A.nslots = nslots;
     A.queue = c->d_queue;
     A.counters = c->d_counters;
@@ -180,14 +209,46 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
 }
 
 me_status invert_dispatch(me_ctx *c, const void *d_prof, int is_f64, int64_t n, double *d_res, double *d_fit,
-                          const double *d_init, int learned, int32_t *d_px, me_stats *st)
+                          const double *d_init, int learned, int32_t *d_px, me_stats *st,
+                          double *ext_scratch = nullptr)
 {
     if (c->cfg.precision == ME_FP32) {
-        return is_f64 ? launch_invert<float, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st)
-                      : launch_invert<float, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st);
+        return is_f64 ? launch_invert<float, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st, ext_scratch)
+                      : launch_invert<float, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st, ext_scratch);
     }
-    return is_f64 ? launch_invert<double, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st)
-                  : launch_invert<double, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st);
+    return is_f64 ? launch_invert<double, double>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st, ext_scratch)
+                  : launch_invert<double, float>(c, d_prof, n, d_res, d_fit, d_init, learned, d_px, st, ext_scratch);
+}
+
+// ---- the allocation-free entry point's scratch
+// The per-lane scratch of the inversion kernel (gradient + two 9x9 matrix
+// slots, me::SL_N doubles per lane, structure of arrays) for at most
+// me::kNoAllocLanes resident lanes, as a static device array: each GPU the
+// library runs on holds its own copy (SL_N * kNoAllocLanes * 8 B = 89.6 MB)
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (on the host it is only a stand-in for the symbol): its device
+// address is resolved per device by hipGetSymbolAddress and kept here.
+__device__ __align__(256) double scratch[(size_t)me::SL_N * me::kNoAllocLanes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// resolves the array's address on the current device once; called by
+// me_ctx_create (set-up) and, should that have failed, by the entry point
+hipError_t noalloc_scratch(int device, double **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double *res, double *fit,
@@ -420,6 +481,23 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
         return hip_status(e);
     }
     me::mlp_init(c->mlp, 4 * nl);
+    // the allocation-free entry point's static scratch on this device: its
+    // address is resolved here, at set-up (a failure is reported by that entry
+    // point, which retries)
+    double *noalloc = nullptr;
+    (void)noalloc_scratch(c->device, &noalloc);
+    // one empty launch of each inversion kernel this context can run (float32
+    // and float64 profiles): the HIP runtime reserves a kernel's register-spill
+    // memory at its first launch in a process, which happens here, at set-up,
+    // and not inside a later inversion call
+    for (int f64 = 0; f64 < 2; f64++) {
+        me_status ws = invert_dispatch(c, c->d_lam, f64, 0, c->d_lam, nullptr, nullptr, 0, nullptr, nullptr);
+        if (ws == ME_OK && hipStreamSynchronize(c->stream) != hipSuccess) ws = ME_EDEVICE;
+        if (ws != ME_OK) {
+            me_ctx_destroy(c);
+            return ws;
+        }
+    }
     *out = c;
     return ME_OK;
 }
@@ -517,6 +595,30 @@ me_status me_invert_device(me_ctx *c, const void *d_prof, int prof_is_f64, int64
         for (int k = 0; k < 4; k++)
             if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
     }
+    HIPCHK(hipEventRecord(c->et1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st) st->total_ms = ms_between(c->et0, c->et1);
+    return ME_OK;
+}
+
+me_status me_invert_device_noalloc(me_ctx *c, const void *d_prof, int prof_is_f64, int64_t n, double *d_res,
+                                   const double *d_init, me_stats *st)
+{
+    if (!c || n < 0 || (n > 0 && (!d_prof || !d_res))) return ME_EINVAL;
+    const int learned = c->cfg.init_mode == ME_INIT_LEARNED;
+    // the learned prediction and per-pixel statistics would need buffers
+    if (learned && !d_init) return ME_EINVAL;
+    if (st && (st->iters || st->runs || st->trials || st->status)) return ME_EINVAL;
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (n == 0) return ME_OK;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->device) return ME_EINVAL;
+    double *scr = nullptr;
+    HIPCHK(noalloc_scratch(dev, &scr));
+    HIPCHK(hipEventRecord(c->et0, c->stream));
+    me_status s = invert_dispatch(c, d_prof, prof_is_f64, n, d_res, nullptr, d_init, learned, nullptr, st, scr);
+    if (s != ME_OK) return s;
     HIPCHK(hipEventRecord(c->et1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (st) st->total_ms = ms_between(c->et0, c->et1);

@@ -27,6 +27,10 @@ enum { SL_BEST = 0, SL_GUESS = 9, SL_LEARN = 18, SL_CHI_BEST = 27, SL_CHI_LEARN 
 // then two 81-entry matrices: the trial's input A and its left factor U
 // alternate between them (mbuf selects the input)
 enum { SL_G = 0, SL_MAT = 9, SL_N = 9 + 162 };
+// lanes of the allocation-free entry point's static scratch: 256 CUs x 256
+// resident lanes (the kernel holds one 256-lane block, i.e. one wave per SIMD,
+// per CU)
+constexpr int64_t kNoAllocLanes = 65536;
 
 struct InvArgs {
     const void *prof;

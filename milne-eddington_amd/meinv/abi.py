@@ -93,6 +93,7 @@ ENTRY_POINTS = [
     ("me_invert_batch", C.c_int, [_vp, _fp, _i64, _dp, _dp, _dp, C.POINTER(MeStats)]),
     ("me_invert_batch_f64", C.c_int, [_vp, _dp, _i64, _dp, _dp, _dp, C.POINTER(MeStats)]),
     ("me_invert_device", C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _vp, _vp, C.POINTER(MeStats)]),
+    ("me_invert_device_noalloc", C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _vp, C.POINTER(MeStats)]),
     ("me_synth_batch", C.c_int, [_vp, _dp, _i64, _dp, _dp]),
     ("me_gen_map_device", C.c_int, [_vp, C.c_uint64, _dp, C.c_double, C.c_double, C.c_double,
                                     _i64, _vp, _vp]),

@@ -161,6 +161,17 @@ class Inversion:
                "me_invert_device")
         return Stats(st, None)
 
+    def invert_device_noalloc(self, d_prof: int, n: int, d_res: int, prof_is_f64: bool = False,
+                              d_init: int | None = None):
+        """me_invert_device_noalloc: the same inversion with the library's
+        static scratch, allocating nothing (no fit output, no per-pixel
+        statistics; learned mode needs d_init)."""
+        st = MeStats()
+        _check(self.lib.me_invert_device_noalloc(self._h, C.c_void_p(d_prof), int(prof_is_f64), int(n),
+                                                 C.c_void_p(d_res), C.c_void_p(d_init) if d_init else None,
+                                                 C.byref(st)), "me_invert_device_noalloc")
+        return Stats(st, None)
+
     # -- synthetic maps and the learned initialiser
     def gen_map_device(self, seed: int, ranges, n: int, d_prof: int, d_truth: int | None = None,
                        ff_frac: float = 0.0, ff_bmax: float = 50.0, sigma: float = 1e-3):
