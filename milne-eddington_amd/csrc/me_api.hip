@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/meinv.h"
@@ -51,6 +53,10 @@ struct me_ctx {
     unsigned long long *d_queue = nullptr, *d_counters = nullptr;
     DevCfg *d_cfg = nullptr;  // device copy of dc, read by the inversion kernel
     DevBuf scratch, prof, res, fit, init, pxs, par, out, jac, fitpar;
+    // pinned staging blocks of me_invert_batch's host copies and the events
+    // of their last DMA (kStageThreads x 2)
+    void *pin[4][2] = {};
+    hipEvent_t pev[4][2] = {};
     me::MlpState mlp;
 };
 
@@ -120,26 +126,56 @@ void persistent_grid(const me_ctx *c, K kernel, int64_t n, int &block, int &grid
     grid = (int)std::max<int64_t>(1, std::min(g, need));
 }
 
-// clears the work counter and the event counters of the no-alloc path (a
-// fill kernel rather than hipMemsetAsync, which graph capture records badly)
-__global__ void zero_u64_kernel(unsigned long long *p, int n)
+// sets the work counter and clears the event counters of a launch (a fill
+// kernel rather than hipMemsetAsync, which graph capture records badly)
+__global__ void fill_u64_kernel(unsigned long long *p, int n, unsigned long long v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    if (i < n) p[i] = v;
 }
 
-// ext_scratch: the no-alloc path's static scratch array (device address);
-// null: the context's own buffer, grown on demand
+// where one inversion launch runs: its stream, per-lane scratch (a static
+// array, or a context buffer grown on demand), work counter, event counters
+// and the events around the kernel
+struct Site {
+    hipStream_t stream;
+    double *ext_scratch;   // the no-alloc path's static array, or null
+    DevBuf *scratch;       // grown on demand when ext_scratch is null
+    unsigned long long *queue, *counters;
+    hipEvent_t ev0, ev1;
+};
+
+void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
+{
+    st->n_pixels = n;
+    st->n_valid = (int64_t)ct[me::CT_VALID];
+    st->n_converged = (int64_t)ct[me::CT_CONV];
+    st->n_runs = (int64_t)ct[me::CT_RUNS];
+    st->n_iters = (int64_t)ct[me::CT_ITERS];
+    st->n_jac = (int64_t)ct[me::CT_JAC];
+    st->n_trial = (int64_t)ct[me::CT_TRIAL];
+    st->n_hess = (int64_t)ct[me::CT_HESS];
+    st->n_svd = (int64_t)ct[me::CT_SVD];
+    st->n_bk = (int64_t)ct[me::CT_BK];
+    for (int b = 0; b < 8; b++) st->n_fadd[b] = (int64_t)ct[me::CT_F0 + b];
+    st->n_svd_fail = (int64_t)ct[me::CT_SVDFAIL];
+    st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
+}
+
+// Enqueues one persistent launch over the pixels [p0, p1) of a batch of npx
+// pixels on S.stream, without waiting for it.  Pixel indices stay those of the
+// batch: records, per-pixel arrays and fitpar are indexed by them, the
+// previous pixel that Init_Guess may read at small nl (H7) is the batch's, and
+// restart draws are keyed by them; the work counter starts at p0.
 template <typename T, typename TIn>
-me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res, double *d_fit,
-                        const double *d_init, int learned, int32_t *d_px, me_stats *st,
-                        double *ext_scratch = nullptr)
+me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p0, int64_t p1, int64_t npx,
+                         double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px)
 {
     auto kern = me::invert_kernel<T, TIn>;
     int block, grid;
-    persistent_grid(c, kern, n, block, grid);
+    persistent_grid(c, kern, p1 - p0, block, grid);
     double *scratch = nullptr;
-    if (ext_scratch) {
+    if (S.ext_scratch) {
         const int64_t cap = me::kNoAllocLanes / block;
         if (grid > cap) {
             // an explicit launch shape beyond the static array is an error; the
@@ -147,63 +183,68 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
             if (c->cfg.waves_per_cu > 0 || c->cfg.block_size > 0) return ME_EINVAL;
             grid = (int)cap;
         }
-        scratch = ext_scratch;
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_queue, 1);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_counters, (int)me::CT_N);
-        HIPCHK(hipGetLastError());
+        scratch = S.ext_scratch;
     } else {
-        HIPCHK(c->scratch.ensure(sizeof(double) * me::SL_N * (int64_t)grid * block));
-        scratch = (double *)c->scratch.p;
-        HIPCHK(hipMemsetAsync(c->d_queue, 0, sizeof(unsigned long long), c->stream));
-        HIPCHK(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * me::CT_N, c->stream));
+        HIPCHK(S.scratch->ensure(sizeof(double) * me::SL_N * (int64_t)grid * block));
+        scratch = (double *)S.scratch->p;
     }
-    const int64_t nslots = (int64_t)grid * block;
-    if (d_fit) HIPCHK(c->fitpar.ensure(sizeof(double) * 9 * n));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, S.stream, S.queue, 1, (unsigned long long)p0);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, S.stream, S.counters, (int)me::CT_N, 0ull);
+    HIPCHK(hipGetLastError());
     me::InvArgs A;
     A.prof = d_prof;
-    A.n = n;
+    A.n = p1;
     A.pix_base = c->pix_base;
     A.res = d_res;
-    A.fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
+    A.fitpar = d_fitpar;
     A.init = d_init;
     A.learned = learned;
     A.scratch = scratch;
-    A.nslots = nslots;
-    A.queue = c->d_queue;
-    A.counters = c->d_counters;
+    A.nslots = (int64_t)grid * block;
+    A.queue = S.queue;
+    A.counters = S.counters;
     A.px_iters = d_px;
-    A.px_runs = d_px ? d_px + n : nullptr;
-    A.px_trials = d_px ? d_px + 2 * n : nullptr;
-    A.px_status = d_px ? d_px + 3 * n : nullptr;
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg, A);
+    A.px_runs = d_px ? d_px + npx : nullptr;
+    A.px_trials = d_px ? d_px + 2 * npx : nullptr;
+    A.px_status = d_px ? d_px + 3 * npx : nullptr;
+    if (S.ev0) HIPCHK(hipEventRecord(S.ev0, S.stream));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, S.stream, (const DevCfg *)c->d_cfg, A);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    if (d_fit) {
-        const int fb = 256, fg = (int)((n + fb - 1) / fb);
-        hipLaunchKernelGGL(me::fit_kernel<T>, dim3(fg), dim3(fb), 0, c->stream, c->dc, (const double *)c->fitpar.p, n,
-                           d_fit);
-        HIPCHK(hipGetLastError());
-    }
+    if (S.ev1) HIPCHK(hipEventRecord(S.ev1, S.stream));
+    return ME_OK;
+}
+
+template <typename T>
+me_status enqueue_fit(me_ctx *c, hipStream_t stream, const double *fitpar, int64_t p0, int64_t p1, double *d_fit)
+{
+    const int64_t m = p1 - p0;
+    const int fb = 256, fg = (int)((m + fb - 1) / fb);
+    hipLaunchKernelGGL(me::fit_kernel<T>, dim3(fg), dim3(fb), 0, stream, c->dc, fitpar + p0 * 9, m,
+                       d_fit + p0 * 4 * c->nl);
+    return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
+}
+
+// one launch over the whole batch on the context stream, waited for
+// ext_scratch: the no-alloc path's static scratch array (device address);
+// null: the context's own buffer, grown on demand
+template <typename T, typename TIn>
+me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res, double *d_fit,
+                        const double *d_init, int learned, int32_t *d_px, me_stats *st,
+                        double *ext_scratch = nullptr)
+{
+    if (d_fit) HIPCHK(c->fitpar.ensure(sizeof(double) * 9 * n));
+    double *fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
+    const Site S{c->stream, ext_scratch, &c->scratch, c->d_queue, c->d_counters, c->ev0, c->ev1};
+    me_status s = enqueue_invert<T, TIn>(c, S, d_prof, 0, n, n, d_res, fitpar, d_init, learned, d_px);
+    if (s != ME_OK) return s;
+    if (d_fit && (s = enqueue_fit<T>(c, c->stream, fitpar, 0, n, d_fit)) != ME_OK) return s;
     unsigned long long ct[me::CT_N];
     HIPCHK(hipMemcpyAsync(ct, c->d_counters, sizeof(ct), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (st) {
         st->kernel_ms = ms_between(c->ev0, c->ev1);
-        st->n_pixels = n;
-        st->n_valid = (int64_t)ct[me::CT_VALID];
-        st->n_converged = (int64_t)ct[me::CT_CONV];
-        st->n_runs = (int64_t)ct[me::CT_RUNS];
-        st->n_iters = (int64_t)ct[me::CT_ITERS];
-        st->n_jac = (int64_t)ct[me::CT_JAC];
-        st->n_trial = (int64_t)ct[me::CT_TRIAL];
-        st->n_hess = (int64_t)ct[me::CT_HESS];
-        st->n_svd = (int64_t)ct[me::CT_SVD];
-        st->n_bk = (int64_t)ct[me::CT_BK];
-        for (int b = 0; b < 8; b++) st->n_fadd[b] = (int64_t)ct[me::CT_F0 + b];
-        st->n_svd_fail = (int64_t)ct[me::CT_SVDFAIL];
-        st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
+        fill_stats(st, ct, n);
     }
     return ME_OK;
 }
@@ -251,40 +292,105 @@ hipError_t noalloc_scratch(int device, double **out)
     return hipSuccess;
 }
 
-me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double *res, double *fit,
-                      const double *init, me_stats *st)
+// Copies between the caller's pageable host buffers and device memory go
+// through pinned staging blocks: each of kStageThreads host threads moves
+// every kStageThreads-th block of kStageBytes with two pinned blocks of its
+// own, so that its memcpy of one block overlaps the DMA of the other and the
+// threads' memcpys overlap each other (a pageable hipMemcpy runs the host
+// side on one thread, in small pieces).  All DMAs go on the context stream.
+constexpr int kStageThreads = 4;
+constexpr size_t kStageBytes = size_t(8) << 20;
+
+me_status staging_buffers(me_ctx *c)
 {
-    if (!c || n < 0 || (n > 0 && (!prof || !res))) return ME_EINVAL;
-    const int learned = c->cfg.init_mode == ME_INIT_LEARNED;
-    if (learned && !init && !c->mlp.trained) return ME_ENOTREADY;
-    if (st) {
-        int32_t *keep[4] = {st->iters, st->runs, st->trials, st->status};
-        std::memset(st, 0, sizeof(*st));
-        st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
-    }
-    if (n == 0) return ME_OK;
-    HIPCHK(hipEventRecord(c->et0, c->stream));
+    for (int t = 0; t < kStageThreads; t++)
+        for (int j = 0; j < 2; j++) {
+            if (!c->pin[t][j]) HIPCHK(hipHostMalloc(&c->pin[t][j], kStageBytes, hipHostMallocDefault));
+            if (!c->pev[t][j]) HIPCHK(hipEventCreateWithFlags(&c->pev[t][j], hipEventDisableTiming));
+        }
+    return ME_OK;
+}
+
+me_status staged_copy(me_ctx *c, void *dst, const void *src, size_t bytes, bool to_device)
+{
+    if (bytes == 0) return ME_OK;
+    me_status s = staging_buffers(c);
+    if (s != ME_OK) return s;
+    const size_t nb = (bytes + kStageBytes - 1) / kStageBytes;
+    std::atomic<int> failed{0};
+    auto block = [&](size_t b, size_t &off, size_t &len) {
+        off = b * kStageBytes;
+        len = std::min(kStageBytes, bytes - off);
+    };
+    auto worker = [&](int t) {
+        size_t off, len;
+        if (to_device) {
+            int j = 0;
+            for (size_t b = t; b < nb && !failed; b += kStageThreads, j ^= 1) {
+                block(b, off, len);
+                // the DMA that last read this pinned block has finished
+                if (hipEventSynchronize(c->pev[t][j]) != hipSuccess) { failed = 1; return; }
+                std::memcpy(c->pin[t][j], (const char *)src + off, len);
+                if (hipMemcpyAsync((char *)dst + off, c->pin[t][j], len, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                    hipEventRecord(c->pev[t][j], c->stream) != hipSuccess) { failed = 1; return; }
+            }
+        } else {
+            // the DMA of the thread's next block is in flight while it copies
+            // the current one out
+            auto fetch = [&](size_t b, int j) {
+                size_t o, l;
+                block(b, o, l);
+                return hipMemcpyAsync(c->pin[t][j], (const char *)src + o, l, hipMemcpyDeviceToHost, c->stream) ==
+                           hipSuccess &&
+                       hipEventRecord(c->pev[t][j], c->stream) == hipSuccess;
+            };
+            if ((size_t)t < nb && !fetch(t, 0)) { failed = 1; return; }
+            int j = 0;
+            for (size_t b = t; b < nb && !failed; b += kStageThreads, j ^= 1) {
+                if (b + kStageThreads < nb && !fetch(b + kStageThreads, j ^ 1)) { failed = 1; return; }
+                if (hipEventSynchronize(c->pev[t][j]) != hipSuccess) { failed = 1; return; }
+                block(b, off, len);
+                std::memcpy((char *)dst + off, c->pin[t][j], len);
+            }
+        }
+    };
+    const int nt = (int)std::min<size_t>(kStageThreads, nb);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(worker, t);
+    worker(0);
+    for (std::thread &th : pool) th.join();
+    return failed ? ME_EDEVICE : ME_OK;
+}
+
+template <typename T, typename TIn>
+me_status invert_host_t(me_ctx *c, const TIn *prof, int64_t n, double *res, double *fit, const double *init,
+                        int learned, me_stats *st)
+{
     const int nl = c->nl;
-    HIPCHK(c->prof.ensure(esz * 4 * nl * n));
+    HIPCHK(hipEventRecord(c->et0, c->stream));
+    HIPCHK(c->prof.ensure(sizeof(TIn) * 4 * nl * n));
     HIPCHK(c->res.ensure(sizeof(double) * 10 * n));
     if (fit) HIPCHK(c->fit.ensure(sizeof(double) * 4 * nl * n));
     if (init || learned) HIPCHK(c->init.ensure(sizeof(double) * 9 * n));
     const bool want_px = st && (st->iters || st->runs || st->trials || st->status);
     if (want_px) HIPCHK(c->pxs.ensure(sizeof(int32_t) * 4 * n));
-    HIPCHK(hipMemcpyAsync(c->prof.p, prof, esz * 4 * nl * n, hipMemcpyHostToDevice, c->stream));
+    me_status s = staged_copy(c, c->prof.p, prof, sizeof(TIn) * 4 * nl * n, true);
+    if (s != ME_OK) return s;
     if (init) {
         // explicit starting points; in learned mode they take the network's place
-        HIPCHK(hipMemcpyAsync(c->init.p, init, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->stream));
+        s = staged_copy(c, c->init.p, init, sizeof(double) * 9 * n, true);
+        if (s != ME_OK) return s;
     } else if (learned) {
-        me_status s = me::mlp_predict(c->mlp, c->dc, c->stream, c->prof.p, esz == 8, n, (double *)c->init.p);
+        s = me::mlp_predict(c->mlp, c->dc, c->stream, c->prof.p, sizeof(TIn) == 8, n, (double *)c->init.p);
         if (s != ME_OK) return s;
     }
-    me_status s = invert_dispatch(c, c->prof.p, esz == 8, n, (double *)c->res.p, fit ? (double *)c->fit.p : nullptr,
-                                  (init || learned) ? (const double *)c->init.p : nullptr, learned,
-                                  want_px ? (int32_t *)c->pxs.p : nullptr, st);
+    s = launch_invert<T, TIn>(c, c->prof.p, n, (double *)c->res.p, fit ? (double *)c->fit.p : nullptr,
+                              (init || learned) ? (const double *)c->init.p : nullptr, learned,
+                              want_px ? (int32_t *)c->pxs.p : nullptr, st);
     if (s != ME_OK) return s;
-    HIPCHK(hipMemcpyAsync(res, c->res.p, sizeof(double) * 10 * n, hipMemcpyDeviceToHost, c->stream));
-    if (fit) HIPCHK(hipMemcpyAsync(fit, c->fit.p, sizeof(double) * 4 * nl * n, hipMemcpyDeviceToHost, c->stream));
+    s = staged_copy(c, res, c->res.p, sizeof(double) * 10 * n, false);
+    if (s == ME_OK && fit) s = staged_copy(c, fit, c->fit.p, sizeof(double) * 4 * nl * n, false);
+    if (s != ME_OK) return s;
     std::vector<int32_t> px;
     if (want_px) {
         px.resize(4 * n);
@@ -299,6 +405,25 @@ me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double
             if (dst[k]) std::memcpy(dst[k], px.data() + k * n, sizeof(int32_t) * n);
     }
     return ME_OK;
+}
+
+me_status invert_host(me_ctx *c, const void *prof, size_t esz, int64_t n, double *res, double *fit,
+                      const double *init, me_stats *st)
+{
+    if (!c || n < 0 || (n > 0 && (!prof || !res))) return ME_EINVAL;
+    const int learned = c->cfg.init_mode == ME_INIT_LEARNED;
+    if (learned && !init && !c->mlp.trained) return ME_ENOTREADY;
+    if (st) {
+        int32_t *keep[4] = {st->iters, st->runs, st->trials, st->status};
+        std::memset(st, 0, sizeof(*st));
+        st->iters = keep[0]; st->runs = keep[1]; st->trials = keep[2]; st->status = keep[3];
+    }
+    if (n == 0) return ME_OK;
+    if (c->cfg.precision == ME_FP32)
+        return esz == 8 ? invert_host_t<float, double>(c, (const double *)prof, n, res, fit, init, learned, st)
+                        : invert_host_t<float, float>(c, (const float *)prof, n, res, fit, init, learned, st);
+    return esz == 8 ? invert_host_t<double, double>(c, (const double *)prof, n, res, fit, init, learned, st)
+                    : invert_host_t<double, float>(c, (const float *)prof, n, res, fit, init, learned, st);
 }
 
 }  // namespace
@@ -512,6 +637,12 @@ void me_ctx_destroy(me_ctx *c)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar};
     for (DevBuf *b : bufs) b->release();
+    for (auto &row : c->pin)
+        for (void *p : row)
+            if (p) (void)hipHostFree(p);
+    for (auto &row : c->pev)
+        for (hipEvent_t e : row)
+            if (e) (void)hipEventDestroy(e);
     me::mlp_free(c->mlp);
     hipEvent_t evs[] = {c->ev0, c->ev1, c->et0, c->et1};
     for (hipEvent_t e : evs)
