@@ -64,15 +64,65 @@ enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
 //   EPI_F32         Z[z] = C * scale, the partial product of K chunk z =
 //                   blockIdx.z (K split over gridDim.z); Z[z] starts at
 //                   z * zs, and sum_partials adds them in chunk order
-template <int EPI>
+// Operand layouts of the staging loads: LAY_ANY reads element by element
+// through the strides; LAY_K: the 8 consecutive k of a row (A) or column (B)
+// are contiguous in memory; LAY_MN: 8 consecutive m (A) or n (B) of one k are
+// contiguous.  The contiguous forms move 16 bytes per load (the caller
+// guarantees 16-byte alignment) and fall back to single elements on a tile's
+// ragged edge.
+enum { LAY_ANY = 0, LAY_K = 1, LAY_MN = 2 };
+
+template <int LAY>
+__device__ __forceinline__ void stage_tile(__bf16 (*sT)[40], const __bf16 *__restrict__ P, int64_t s_mn, int64_t s_k,
+                                           int mn0, int MN, int k0, int kend, int tid)
+{
+    if (LAY == LAY_K) {
+        // thread: row r (m or n), k group kk .. kk+7
+        const int r = tid >> 2, kk = (tid & 3) * 8;
+        const int mn = mn0 + r, k = k0 + kk;
+        bf16x8 v;
+        if (mn < MN && k + 7 < kend) {
+            v = *(const bf16x8 *)(P + (int64_t)mn * s_mn + k);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                v[q] = (mn < MN && k + q < kend) ? P[(int64_t)mn * s_mn + (int64_t)(k + q) * s_k] : to_bf16(0.f);
+        }
+        *(bf16x8 *)&sT[r][kk] = v;
+    } else if (LAY == LAY_MN) {
+        // thread: k index kk, rows r .. r+7
+        const int kk = tid >> 3, r = (tid & 7) * 8;
+        const int mn = mn0 + r, k = k0 + kk;
+        bf16x8 v;
+        if (mn + 7 < MN && k < kend) {
+            v = *(const bf16x8 *)(P + (int64_t)k * s_k + mn);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                v[q] = (mn + q < MN && k < kend) ? P[(int64_t)(mn + q) * s_mn + (int64_t)k * s_k] : to_bf16(0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) sT[r + q][kk] = v[q];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = tid * 8 + e;
+            const int r = idx >> 5, kk = idx & 31;
+            const int mn = mn0 + r, k = k0 + kk;
+            sT[r][kk] = (mn < MN && k < kend) ? P[(int64_t)mn * s_mn + (int64_t)k * s_k] : to_bf16(0.f);
+        }
+    }
+}
+
+template <int EPI, int ALAY = LAY_ANY, int BLAY = LAY_ANY>
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
                                                  const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn, int M, int N,
                                                  int K, float *__restrict__ Zf, __bf16 *__restrict__ Yb, int ldc,
                                                  const float *__restrict__ bias, const float *__restrict__ aux,
                                                  float scale, int64_t zs)
 {
-    __shared__ __bf16 sA[64][32 + 8];
-    __shared__ __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
+    __shared__ __attribute__((aligned(16))) __bf16 sA[64][32 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
     // split K over gridDim.z (weight gradients: K is the batch)
@@ -83,26 +133,16 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        // 64x32 tiles of A and B^T: 2048 elements each, 8 per thread
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int idx = tid * 8 + e;
-            const int r = idx >> 5, kk = idx & 31;
-            const int m = m0 + r, n = n0 + r, k = k0 + kk;
-            sA[r][kk] = (m < M && k < kend) ? A[(int64_t)m * sam + (int64_t)k * sak] : to_bf16(0.f);
-            sB[r][kk] = (n < N && k < kend) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : to_bf16(0.f);
-        }
+        // 64x32 tiles of A and B^T
+        stage_tile<ALAY>(sA, A, sam, sak, m0, M, k0, kend, tid);
+        stage_tile<BLAY>(sB, B, sbn, sbk, n0, N, k0, kend, tid);
         __syncthreads();
         // lane l: row/col (l % 16), k group 8*(l / 16) .. +7
         const int rr = lane & 15, kg = (lane >> 4) * 8;
-        bf16x8 a;
-#pragma unroll
-        for (int q = 0; q < 8; q++) a[q] = sA[wave * 16 + rr][kg + q];
+        const bf16x8 a = *(const bf16x8 *)&sA[wave * 16 + rr][kg];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            bf16x8 b;
-#pragma unroll
-            for (int q = 0; q < 8; q++) b[q] = sB[t * 16 + rr][kg + q];
+            const bf16x8 b = *(const bf16x8 *)&sB[t * 16 + rr][kg];
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
         }
         __syncthreads();
@@ -134,17 +174,24 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
     }
 }
 
-// profiles -> normalised bf16 inputs [n][nin_pad]
+// profiles -> normalised bf16 inputs [n][nin_pad]: one wave per pixel (the
+// I maximum by a wave reduction, each lane writes every 64th input), so the
+// loads and the bf16 stores are coalesced whatever nl is
+constexpr int MLP_IN_PX = 4;   // pixels per 256-thread block
 template <typename TIn>
-__global__ void mlp_inputs(const TIn *__restrict__ prof, int64_t n, int nl, int nin_pad, __bf16 *__restrict__ X)
+__global__ __launch_bounds__(256) void mlp_inputs(const TIn *__restrict__ prof, int64_t n, int nl, int nin_pad,
+                                                  __bf16 *__restrict__ X)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const int64_t p = (int64_t)blockIdx.x * MLP_IN_PX + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= n) return;  // wave-uniform
     const TIn *pr = prof + p * 4 * nl;
     float imax = 0.f;
-    for (int i = 0; i < nl; i++) imax = fmaxf(imax, (float)pr[i]);
+    for (int i = lane; i < nl; i += 64) imax = fmaxf(imax, (float)pr[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) imax = fmaxf(imax, __shfl_xor(imax, off));
     const float inv = imax > 0.f ? 1.0f / imax : 0.f;
-    for (int i = 0; i < nin_pad; i++) {
+    for (int i = lane; i < nin_pad; i += 64) {
         float x = 0.f;
         if (i < nl) x = (float)pr[i] * inv - 1.0f;
         else if (i < 4 * nl) x = 10.0f * (float)pr[i] * inv;
@@ -505,6 +552,9 @@ struct MlpState {
     float *Z[MLP_NL] = {nullptr}, *Y = nullptr, *loss = nullptr;
     float *prof_tmp = nullptr;
     double *truth_tmp = nullptr;
+    // the training set's normalised inputs and targets, built once per training
+    __bf16 *Xall = nullptr;
+    float *Yall = nullptr;
     int64_t tmp_cap = 0;
     // fused-inference weights (fragment layout), rebuilt after parameter changes
     bf16x8 *wpk = nullptr;
@@ -542,6 +592,7 @@ inline void mlp_free(MlpState &m)
 {
     dev_release(m.P); dev_release(m.Mo); dev_release(m.Vo); dev_release(m.G); dev_release(m.Gp); dev_release(m.Pb);
     dev_release(m.X); dev_release(m.Y); dev_release(m.loss); dev_release(m.prof_tmp); dev_release(m.truth_tmp);
+    dev_release(m.Xall); dev_release(m.Yall);
     dev_release(m.wpk); dev_release(m.bpk);
     for (int l = 0; l < MLP_NL; l++) { dev_release(m.A[l]); dev_release(m.D[l]); dev_release(m.Z[l]); }
     m = MlpState();
@@ -624,20 +675,20 @@ inline me_status mlp_init_params(MlpState &m, hipStream_t s, uint64_t seed)
 
 inline dim3 gemm_grid(int M, int N) { return dim3((M + 63) / 64, (N + 63) / 64); }
 
-// forward pass over `rows` rows of m.X
-inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
+// forward pass over `rows` rows of X (m.X unless given)
+inline void mlp_forward(MlpState &m, hipStream_t s, int rows, const __bf16 *X = nullptr)
 {
-    const __bf16 *in = m.X;
+    const __bf16 *in = X ? X : m.X;
     for (int l = 0; l < MLP_NL; l++) {
         const int din = m.dims[l], dout = m.dims[l + 1];
         const __bf16 *W = m.Pb + m.offW[l];
         const float *b = m.P + m.offB[l];
         if (l < MLP_NL - 1)
-            hipLaunchKernelGGL(gemm_bf16<EPI_FWD_GELU>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
+            hipLaunchKernelGGL((gemm_bf16<EPI_FWD_GELU, LAY_K, LAY_MN>), gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
                                (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], m.A[l], dout, b,
                                (const float *)nullptr, 1.0f, (int64_t)0);
         else
-            hipLaunchKernelGGL(gemm_bf16<EPI_FWD_LINEAR>, gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
+            hipLaunchKernelGGL((gemm_bf16<EPI_FWD_LINEAR, LAY_K, LAY_MN>), gemm_grid(rows, dout), dim3(256), 0, s, in, (int64_t)din,
                                (int64_t)1, W, (int64_t)dout, (int64_t)1, rows, dout, din, m.Z[l], (__bf16 *)nullptr,
                                dout, b, (const float *)nullptr, 1.0f, (int64_t)0);
         in = m.A[l];
@@ -645,8 +696,9 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows)
 }
 
 // backward pass (loss gradient already in m.D[last]); gradients into m.G
-inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
+inline void mlp_backward(MlpState &m, hipStream_t s, int rows, const __bf16 *X = nullptr)
 {
+    if (!X) X = m.X;
     // every gradient entry is the sum of ksplit chunk partials (weights: the
     // K chunks of the GEMM; biases: row chunks), which sit in Gp at stride np
     // at the entry's own offset; one sum_partials adds them all in chunk
@@ -654,11 +706,11 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
     const int ksplit = rows >= 4096 ? MLP_KSPLIT : (rows >= 1024 ? 8 : 1);
     for (int l = MLP_NL - 1; l >= 0; l--) {
         const int din = m.dims[l], dout = m.dims[l + 1];
-        const __bf16 *act = l == 0 ? m.X : m.A[l - 1];
+        const __bf16 *act = l == 0 ? X : m.A[l - 1];
         // dW = act^T dZ  (M = din, N = dout, K = rows split over gridDim.z)
         dim3 g = gemm_grid(din, dout);
         g.z = ksplit;
-        hipLaunchKernelGGL(gemm_bf16<EPI_F32>, g, dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
+        hipLaunchKernelGGL((gemm_bf16<EPI_F32, LAY_MN, LAY_MN>), g, dim3(256), 0, s, act, (int64_t)1, (int64_t)din,
                            (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, din, dout, rows, m.Gp + m.offW[l],
                            (__bf16 *)nullptr, dout, (const float *)nullptr, (const float *)nullptr, 1.0f, m.np);
         // db = column sums of dZ over ksplit row chunks
@@ -667,7 +719,7 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows)
                            dout, rchunk, m.Gp + m.offB[l], m.np);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W^T) * gelu'(Z_{l-1})
-            hipLaunchKernelGGL(gemm_bf16<EPI_BWD_GELU>, gemm_grid(rows, din), dim3(256), 0, s,
+            hipLaunchKernelGGL((gemm_bf16<EPI_BWD_GELU, LAY_K, LAY_K>), gemm_grid(rows, din), dim3(256), 0, s,
                                (const __bf16 *)m.D[l], (int64_t)dout, (int64_t)1, (const __bf16 *)(m.Pb + m.offW[l]),
                                (int64_t)1, (int64_t)dout, rows, din, dout, (float *)nullptr, m.D[l - 1], din,
                                (const float *)nullptr, (const float *)m.Z[l - 1], 1.0f, (int64_t)0);
@@ -693,8 +745,12 @@ inline me_status mlp_tmp(MlpState &m, int64_t rows, int nl)
     m.tmp_cap = 0;
     dev_release(m.prof_tmp);
     dev_release(m.truth_tmp);
+    dev_release(m.Xall);
+    dev_release(m.Yall);
     hipError_t e = hipMalloc(&m.prof_tmp, sizeof(float) * rows * 4 * nl);
     e = e ? e : hipMalloc(&m.truth_tmp, sizeof(double) * rows * 9);
+    e = e ? e : hipMalloc(&m.Xall, sizeof(__bf16) * rows * m.ninp);
+    e = e ? e : hipMalloc(&m.Yall, sizeof(float) * rows * MLP_OUTP);
     if (e != hipSuccess) return ME_ENOMEM;
     m.tmp_cap = rows;
     return ME_OK;
@@ -717,21 +773,22 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
     mlp_set_ranges(m, ranges);
     if ((st = mlp_init_params(m, s, seed)) != ME_OK) return st;
     if ((st = gen(m.prof_tmp, m.truth_tmp, n_train)) != ME_OK) return st;
+    // normalised inputs and targets of the whole training set, once
+    hipLaunchKernelGGL(mlp_inputs<float>, dim3((unsigned)((n_train + MLP_IN_PX - 1) / MLP_IN_PX)), dim3(256), 0, s,
+                       (const float *)m.prof_tmp, n_train, nl, m.ninp, m.Xall);
+    hipLaunchKernelGGL(mlp_targets, dim3((unsigned)((n_train + 255) / 256)), dim3(256), 0, s,
+                       (const double *)m.truth_tmp, (const float *)m.prof_tmp, n_train, nl, m.norm, m.Yall);
     const int64_t nb = n_train / batch;
     const int64_t total = nb * epochs;
     float b1 = 0.9f, b2 = 0.999f;
     double lossv = 0;
     for (int64_t step = 0; step < total; step++) {
         const int64_t r0 = (step % nb) * batch;
-        hipLaunchKernelGGL(mlp_inputs<float>, dim3((batch + 255) / 256), dim3(256), 0, s,
-                           (const float *)(m.prof_tmp + r0 * 4 * nl), (int64_t)batch, nl, m.ninp, m.X);
-        hipLaunchKernelGGL(mlp_targets, dim3((batch + 255) / 256), dim3(256), 0, s,
-                           (const double *)(m.truth_tmp + r0 * 9), (const float *)(m.prof_tmp + r0 * 4 * nl),
-                           (int64_t)batch, nl, m.norm, m.Y);
-        mlp_forward(m, s, batch);
+        const __bf16 *Xb = m.Xall + r0 * m.ninp;
+        mlp_forward(m, s, batch, Xb);
         hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
-                           (const float *)m.Y, batch, m.D[MLP_NL - 1], m.loss);
-        mlp_backward(m, s, batch);
+                           (const float *)(m.Yall + r0 * MLP_OUTP), batch, m.D[MLP_NL - 1], m.loss);
+        mlp_backward(m, s, batch, Xb);
         // cosine decay to 5% of lr
         const double frac = (double)step / (double)total;
         const float lrs = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(kPi * frac))));
@@ -802,10 +859,10 @@ inline me_status mlp_predict(MlpState &m, const DevCfg &C, hipStream_t s, const 
     for (int64_t r0 = 0; r0 < n; r0 += m.cap) {
         const int rows = (int)((n - r0) < m.cap ? (n - r0) : m.cap);
         if (prof_is_f64)
-            hipLaunchKernelGGL(mlp_inputs<double>, dim3((rows + 255) / 256), dim3(256), 0, s,
+            hipLaunchKernelGGL(mlp_inputs<double>, dim3((rows + MLP_IN_PX - 1) / MLP_IN_PX), dim3(256), 0, s,
                                (const double *)prof + r0 * 4 * nl, (int64_t)rows, nl, m.ninp, m.X);
         else
-            hipLaunchKernelGGL(mlp_inputs<float>, dim3((rows + 255) / 256), dim3(256), 0, s,
+            hipLaunchKernelGGL(mlp_inputs<float>, dim3((rows + MLP_IN_PX - 1) / MLP_IN_PX), dim3(256), 0, s,
                                (const float *)prof + r0 * 4 * nl, (int64_t)rows, nl, m.ninp, m.X);
         mlp_forward(m, s, rows);
         if (prof_is_f64)
