@@ -2,6 +2,7 @@
 // (INV_INIT's derived constants, Src/INV_INIT.c:121-205), device buffers,
 // launches of the synthesis, map and inversion kernels.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -53,6 +54,7 @@ struct me_ctx {
     unsigned long long *d_queue = nullptr, *d_counters = nullptr;
     DevCfg *d_cfg = nullptr;  // device copy of dc, read by the inversion kernel
     DevBuf scratch, prof, res, fit, init, pxs, par, out, jac, fitpar;
+    DevBuf okey, oidx, otmp;  // learned mode's work-queue order: keys, pixel indices, sort storage
     // pinned staging blocks of me_invert_batch's host copies and the events
     // of their last DMA (kStageThreads x 2)
     void *pin[4][2] = {};
@@ -162,6 +164,32 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
     st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
 }
 
+// Learned mode's work-queue order (start_chi_kernel): pixels sorted by the
+// chi2 at their starting point, worst first.  Returns the order (queue
+// position -> pixel) in the context's buffers, or null when n does not fit
+// 32-bit indices.
+template <typename TIn>
+me_status learned_order(me_ctx *c, hipStream_t s, const void *d_prof, const double *d_init, int64_t n,
+                        const int32_t **order)
+{
+    *order = nullptr;
+    if (n > INT_MAX) return ME_OK;
+    HIPCHK(c->okey.ensure(sizeof(float) * 2 * n));
+    HIPCHK(c->oidx.ensure(sizeof(int32_t) * 2 * n));
+    float *key = (float *)c->okey.p;
+    int32_t *idx = (int32_t *)c->oidx.p;
+    const int b = 256;
+    hipLaunchKernelGGL(me::start_chi_kernel<TIn>, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, s, c->dc, d_prof,
+                       d_init, n, key, idx);
+    HIPCHK(hipGetLastError());
+    size_t tmp = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, key, key + n, idx, idx + n, (int)n, 0, 32, s));
+    HIPCHK(c->otmp.ensure(tmp > 0 ? tmp : 1));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(c->otmp.p, tmp, key, key + n, idx, idx + n, (int)n, 0, 32, s));
+    *order = idx + n;
+    return ME_OK;
+}
+
 // Enqueues one persistent launch over the pixels [p0, p1) of a batch of npx
 // pixels on S.stream, without waiting for it.  Pixel indices stay those of the
 // batch: records, per-pixel arrays and fitpar are indexed by them, the
@@ -169,7 +197,8 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
 // restart draws are keyed by them; the work counter starts at p0.
 template <typename T, typename TIn>
 me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p0, int64_t p1, int64_t npx,
-                         double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px)
+                         double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px,
+                         const int32_t *order = nullptr)
 {
     auto kern = me::invert_kernel<T, TIn>;
     int block, grid;
@@ -196,6 +225,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
     A.prof = d_prof;
     A.n = p1;
     A.pix_base = c->pix_base;
+    A.order = order;
     A.res = d_res;
     A.fitpar = d_fitpar;
     A.init = d_init;
@@ -236,7 +266,13 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
     if (d_fit) HIPCHK(c->fitpar.ensure(sizeof(double) * 9 * n));
     double *fitpar = d_fit ? (double *)c->fitpar.p : nullptr;
     const Site S{c->stream, ext_scratch, &c->scratch, c->d_queue, c->d_counters, c->ev0, c->ev1};
-    me_status s = enqueue_invert<T, TIn>(c, S, d_prof, 0, n, n, d_res, fitpar, d_init, learned, d_px);
+    // learned mode hands out the pixels with the worst starting point first
+    // (the allocation-free path keeps index order: sorting needs buffers)
+    const int32_t *order = nullptr;
+    me_status s = ME_OK;
+    if (learned && d_init && !ext_scratch && (s = learned_order<TIn>(c, c->stream, d_prof, d_init, n, &order)) != ME_OK)
+        return s;
+    s = enqueue_invert<T, TIn>(c, S, d_prof, 0, n, n, d_res, fitpar, d_init, learned, d_px, order);
     if (s != ME_OK) return s;
     if (d_fit && (s = enqueue_fit<T>(c, c->stream, fitpar, 0, n, d_fit)) != ME_OK) return s;
     unsigned long long ct[me::CT_N];
@@ -635,7 +671,8 @@ void me_ctx_destroy(me_ctx *c)
     void *owned[] = {c->d_lam, c->d_dlam, c->d_dlamf, c->d_etab, c->d_etabf, c->d_queue, c->d_counters, c->d_cfg};
     for (void *p : owned)
         if (p) (void)hipFree(p);
-    DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar};
+    DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar,
+                      &c->okey, &c->oidx, &c->otmp};
     for (DevBuf *b : bufs) b->release();
     for (auto &row : c->pin)
         for (void *p : row)

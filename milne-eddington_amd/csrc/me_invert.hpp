@@ -36,6 +36,7 @@ struct InvArgs {
     const void *prof;
     int64_t n;
     int64_t pix_base;  // global index of pixel 0 of the batch: the Philox counter of its restart draws
+    const int32_t *order;  // work-queue order of the pixels (queue position -> pixel), or null: index order
     double *res;
     double *fitpar;  // [n][9] parameters of the best fit before HMI_REF (fit profiles are synthesised from them)
     const double *init;
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             const int64_t p = wave_acquire(A.queue, need);
             if (!need) continue;
             if (p >= A.n) { phase = PH_DONE; continue; }
-            pix = p;
+            pix = A.order ? (int64_t)A.order[p] : p;
             const int64_t base = pix * 4 * nl;
             // explicit or learned initial parameters, loaded together with
             // the continuum so that both arrive in one round trip
@@ -921,6 +922,53 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
     }
 #undef CT
 #undef SCR
+}
+
+// Work-queue priority of a pixel in learned mode: chi2 of the FP32 synthesis at
+// the (bounds-checked) starting point, in units of the pixel's noise
+// (Noise_Init's (0.01 mean I)^2).  A poor start predicts a long LM run and a
+// heuristic fallback; handing those pixels out first keeps them from being
+// the last ones running when the queue drains.  Pixels that are not
+// inverted get -1 (last).  Only the order changes, never a pixel's result.
+template <typename TIn>
+__global__ __launch_bounds__(256) void start_chi_kernel(DevCfg C, const void *prof, const double *init, int64_t n,
+                                                        float *key, int32_t *idx)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    idx[p] = (int32_t)p;
+    const int nl = C.nl;
+    const int64_t base = p * 4 * nl;
+    double imax = 0.0, isum = 0.0;
+    for (int iw = 0; iw < nl; iw++) {
+        const double I = ld<TIn>(prof, base + iw);
+        imax = imax < I ? I : imax;
+        isum += I;
+    }
+    const double imean = isum / nl;
+    if (!(imean >= C.icrit)) {
+        key[p] = -1.f;
+        return;
+    }
+    double par[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) par[k] = init[p * 9 + k];
+    bounds_check(C, par, 0.6 * imax, 1.4 * imax);
+    PixK<float> K;
+    pix_consts<float>(C, par, K);
+    float chi = 0.f;
+    for (int iw = 0; iw < nl; iw++) {
+        float o[4];
+        me_point<float, false>(C, K, 0.f, iw, o, NoSink{}, (FCount *)nullptr);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float d = o[q] - (float)ld<TIn>(prof, base + q * nl + iw);
+            chi += d * d * (float)C.wsq[q];
+        }
+    }
+    const double nz = 1e-2 * imean;
+    const float k = (float)(chi / (nz * nz * 4.0 * nl));
+    key[p] = isfinite(k) ? k : 3.0e38f;
 }
 
 // forward synthesis, one pixel per thread
