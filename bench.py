@@ -10,7 +10,10 @@ the timed steps run without the FP32 path's per-call Faddeeva region
 counting, and one extra untimed step with it supplies the event counts.
 
 Beside the headline (details):
-  * plain random-restart LM on the same map and GPU (the learned-vs-plain ratio);
+  * plain random-restart LM on the same map and GPU (the learned-vs-plain ratio),
+    also at the other restart budgets of num_run 1, 5 and 10 (the headline runs
+    the workload's: 5 for hmi4k and niris1k, where configurations 1 and 3 put
+    Philox restarts);
   * the reference's own precision (FP64 synthesis, --no-fp64 skips it);
   * the drop-in path from host buffers (me_invert_batch: staging + kernel +
     records back; --no-e2e skips it);
@@ -73,7 +76,7 @@ def roofline(st, nl, fp64, kernel_s, traffic=None):
     return out
 
 
-def hbm_traffic(workload: str, mode: str, precision: str):
+def hbm_traffic(workload: str, mode: str, precision: str, num_run: int):
     """HBM bytes per inversion launch from the committed rocprofv3 FETCH_SIZE
     / WRITE_SIZE summary (scaled by the plain-copy calibration), with the
     file and the kernel build it was measured on; None when not measured."""
@@ -85,8 +88,8 @@ def hbm_traffic(workload: str, mode: str, precision: str):
         if line.startswith("# build:"):
             build = line[len("# build:"):].strip()
         f = line.split()
-        if len(f) >= 5 and f[:3] == [workload, mode, precision]:
-            return float(f[3]) + float(f[4]), f"{TRAFFIC_FILE} ({build})" if build else TRAFFIC_FILE
+        if len(f) >= 6 and f[:4] == [workload, mode, precision, str(num_run)]:
+            return float(f[4]) + float(f[5]), f"{TRAFFIC_FILE} ({build})" if build else TRAFFIC_FILE
     return None
 
 
@@ -184,7 +187,7 @@ def main():
     ap.add_argument("--workload", default="hmi4k")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--pixels", type=int, default=0, help="invert only the first N pixels of the map")
-    ap.add_argument("--num-run", type=int, default=0)
+    ap.add_argument("--num-run", type=int, default=0, help="LM restart budget (default: the workload's, 5 for hmi4k)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
@@ -197,6 +200,8 @@ def main():
     ap.add_argument("--save-mlp", default="", help="write the trained network to this file")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 (reference precision) legs")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer me_invert_batch leg")
+    ap.add_argument("--restarts", type=int, default=1,
+                    help="also time plain and learned at the other num_run of 1, 5, 10 (0 skips)")
     ap.add_argument("--oracle-pixels", type=int, default=16384,
                     help="pixels of the learned-vs-oracle check (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -219,8 +224,7 @@ def main():
     n = w.n if args.pixels <= 0 else min(args.pixels, w.n)
     fp64 = args.precision == "fp64"
     over = dict(precision=ME_FP64 if fp64 else ME_FP32, block_size=args.block, waves_per_cu=args.waves_per_cu)
-    if args.num_run:
-        over["num_run"] = args.num_run
+    over["num_run"] = args.num_run if args.num_run else w.bench_num_run
     cfg = w.config(**over)
     inv = Inversion(cfg, w.wavelengths())
     d_prof = torch.empty((n, 4, w.nl), dtype=torch.float32, device="cuda:0")
@@ -234,7 +238,7 @@ def main():
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     log(f"bench: {w.name} n={n} nl={w.nl} {args.precision} num_run={cfg.num_run} mode={args.mode}")
 
-    def timed(engine, label, steps, warmup, fp64_path, tune_ok=True):
+    def timed(engine, label, steps, warmup, fp64_path, tune_ok=True, count=True):
         # the launch shape is autotuned first (untimed) unless given; timed
         # steps run without the FP32 path's per-call region counting; one extra
         # untimed step with it gives the event counts of the op model (the
@@ -259,8 +263,10 @@ def main():
                 f"{st.n_converged}/{st.n_valid}")
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        engine.set_region_counts(True)
-        counted = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+        counted = None
+        if count:
+            engine.set_region_counts(True)
+            counted = engine.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
         return wall / steps * 1e3, sts, counted, tune
 
     def vs_truth(res):
@@ -337,7 +343,7 @@ def main():
     value = conv / (ms_per_step / 1e3)
     kernel_s = np.mean([s.kernel_ms for s in sts]) / 1e3
     head_res = learned_res if headline == "learned" else d_res.cpu().numpy()
-    line_roof = roofline(counted, w.nl, fp64, kernel_s, hbm_traffic(w.name, headline, args.precision))
+    line_roof = roofline(counted, w.nl, fp64, kernel_s, hbm_traffic(w.name, headline, args.precision, cfg.num_run))
 
     # ---- the reference's own precision: FP64 synthesis, the same network
     if not fp64 and not args.no_fp64:
@@ -355,6 +361,33 @@ def main():
             results["learned_fp64"] = summary(ms, s64)
             results["learned_fp64"]["roofline_frac"] = roofline(c64, w.nl, True, s64[-1].kernel_ms / 1e3)["frac"]
             l64.close()
+
+    # ---- other restart budgets: num_run 1 (no Gaussian jumps), 5 and 10
+    # (configuration 3's bound), plain and learned (the fallback runs num_run),
+    # on the launch shape tuned above
+    if args.restarts:
+        shape = {}
+        for key in ("plain", "learned"):
+            if key in results and "launch" in results[key]:
+                shape[key] = dict(block_size=results[key]["launch"]["block"],
+                                  waves_per_cu=results[key]["launch"]["waves_per_cu"])
+        for nr in [r for r in (1, 5, 10) if r != cfg.num_run]:
+            if "plain" in results:
+                e = Inversion(w.config(**dict(over, num_run=nr, **shape.get("plain", {}))), w.wavelengths())
+                ms, sr, _, _ = timed(e, f"plain num_run={nr}", 1, 0, fp64, tune_ok=False, count=False)
+                results[f"plain_num_run_{nr}"] = summary(ms, sr)
+                e.close()
+            if linv is not None:
+                e = Inversion(w.config(**dict(over, num_run=nr, init_mode=ME_INIT_LEARNED, **shape.get("learned", {}))),
+                              w.wavelengths())
+                e.mlp_import(linv.mlp_params(), w.ranges())
+                ms, sr, _, _ = timed(e, f"learned num_run={nr}", 1, 0, fp64, tune_ok=False, count=False)
+                results[f"learned_num_run_{nr}"] = summary(ms, sr)
+                e.close()
+            if f"plain_num_run_{nr}" in results and f"learned_num_run_{nr}" in results:
+                results[f"speedup_learned_vs_plain_num_run_{nr}"] = round(
+                    results[f"learned_num_run_{nr}"]["converged_pixels_per_s"]
+                    / results[f"plain_num_run_{nr}"]["converged_pixels_per_s"], 3)
 
     # ---- the drop-in path: host buffers in and out (me_invert_batch)
     if not args.no_e2e:

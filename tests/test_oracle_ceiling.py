@@ -3,8 +3,10 @@ bounds: Init_Guess bounds S to [0.6, 1.4] x Imax, the largest observed I of
 the pixel (Src/ME_SOLVER.c:662-666), and on the 6-sample HMI window a wide,
 strong line never reaches the continuum, so some truths have S above 1.4 Imax.
 Starting each pixel's LM run at its truth (the oracle's learned policy fed
-the truths), the converged fraction stays below 99 %, and every pixel that
-fails to converge is one of those."""
+the truths) with num_run = 1, the converged fraction stays below 99 %, and
+every pixel that fails to converge is one of those: without Gaussian restarts
+no run leaves the bounded-S basin.  (Restarts, num_run >= 5, find other fits
+under the criterion for some of them.)"""
 import numpy as np
 
 import oracle as O
