@@ -179,7 +179,7 @@ me_status learned_order(me_ctx *c, hipStream_t s, const void *d_prof, const doub
     float *key = (float *)c->okey.p;
     int32_t *idx = (int32_t *)c->oidx.p;
     const int b = 256;
-    hipLaunchKernelGGL(me::start_chi_kernel<TIn>, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, s, c->dc, d_prof,
+    hipLaunchKernelGGL(me::start_chi_kernel<TIn>, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, s, (const DevCfg *)c->d_cfg, d_prof,
                        d_init, n, key, idx);
     HIPCHK(hipGetLastError());
     size_t tmp = 0;
@@ -250,7 +250,7 @@ me_status enqueue_fit(me_ctx *c, hipStream_t stream, const double *fitpar, int64
 {
     const int64_t m = p1 - p0;
     const int fb = 256, fg = (int)((m + fb - 1) / fb);
-    hipLaunchKernelGGL(me::fit_kernel<T>, dim3(fg), dim3(fb), 0, stream, c->dc, fitpar + p0 * 9, m,
+    hipLaunchKernelGGL(me::fit_kernel<T>, dim3(fg), dim3(fb), 0, stream, (const DevCfg *)c->d_cfg, fitpar + p0 * 9, m,
                        d_fit + p0 * 4 * c->nl);
     return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
 }
@@ -858,10 +858,10 @@ me_status me_synth_batch(me_ctx *c, const double *par, int64_t n, double *stokes
     const int block = 256;
     const int grid = (int)((n + block - 1) / block);
     if (c->cfg.precision == ME_FP32)
-        hipLaunchKernelGGL(me::synth_kernel<float>, dim3(grid), dim3(block), 0, c->stream, c->dc,
+        hipLaunchKernelGGL(me::synth_kernel<float>, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg,
                            (const double *)c->par.p, n, (double *)c->out.p, jac ? (double *)c->jac.p : nullptr);
     else
-        hipLaunchKernelGGL(me::synth_kernel<double>, dim3(grid), dim3(block), 0, c->stream, c->dc,
+        hipLaunchKernelGGL(me::synth_kernel<double>, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg,
                            (const double *)c->par.p, n, (double *)c->out.p, jac ? (double *)c->jac.p : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(stokes, c->out.p, sizeof(double) * 4 * nl * n, hipMemcpyDeviceToHost, c->stream));
@@ -880,7 +880,7 @@ me_status me_gen_map_device(me_ctx *c, uint64_t seed, const double *ranges, doub
     HIPCHK(hipMemcpyAsync(d_rg, ranges, sizeof(double) * 18, hipMemcpyHostToDevice, c->stream));
     const int block = 256;
     const int grid = (int)((n + block - 1) / block);
-    hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, c->dc, seed, (const double *)d_rg,
+    hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg, seed, (const double *)d_rg,
                        ff_frac, ff_bmax, sigma, n, d_prof, d_truth);
     hipError_t e = hipGetLastError();
     hipError_t ef = hipFreeAsync(d_rg, c->stream);
@@ -901,7 +901,7 @@ me_status me_mlp_train(me_ctx *c, uint64_t seed, const double *ranges, double ff
     auto gen = [&](float *prof, double *truth, int64_t n) -> me_status {
         const int block = 256;
         const int grid = (int)((n + block - 1) / block);
-        hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, c->dc, seed, (const double *)d_rg,
+        hipLaunchKernelGGL(me::genmap_kernel, dim3(grid), dim3(block), 0, c->stream, (const DevCfg *)c->d_cfg, seed, (const double *)d_rg,
                            ff_frac, ff_bmax, sigma, n, prof, truth);
         return hipGetLastError() == hipSuccess ? ME_OK : ME_EDEVICE;
     };

@@ -71,9 +71,47 @@ template <> __device__ __forceinline__ float merfc(float x) { return erfcf(x); }
 template <> __device__ __forceinline__ double merfc(double x) { return erfc(x); }
 template <typename T> __device__ __forceinline__ T mabs(T x) { return x < T(0) ? -x : x; }
 
-template <typename T> __device__ __forceinline__ const T *etab_of(const DevCfg &C);
-template <> __device__ __forceinline__ const float *etab_of<float>(const DevCfg &C) { return C.etabf; }
-template <> __device__ __forceinline__ const double *etab_of<double>(const DevCfg &C) { return C.etab; }
+// ---------------------------------------------------------------- pointers
+// The __noinline__ phases receive their pointers as per-lane generic values,
+// so every access would be a flat instruction (counted against both the
+// vector-memory and the LDS/scalar wait counters: an LDS or scalar wait then
+// also waits for outstanding memory traffic) and every read of the
+// wave-uniform configuration a vector load.  The configuration is therefore
+// read through CfgRef, a reference in the read-only (constant) address space
+// whose address sits in scalar registers: reads at uniform offsets are scalar
+// loads through the scalar cache.  uniform_ro does the same for a
+// wave-uniform read-only table, global_ptr types a per-lane pointer as global
+// memory (vector loads and stores that wait on the vector-memory counter
+// only).  The address-space types are kept to the access: a pointer cast to an
+// address space and back to a generic one is folded back by the compiler.
+#define ME_AS_CONST __attribute__((address_space(4)))
+#define ME_AS_GLOBAL __attribute__((address_space(1)))
+template <typename P> using cptr = const ME_AS_CONST P *;
+template <typename P> using gptr = ME_AS_GLOBAL P *;
+using CfgRef = const ME_AS_CONST DevCfg &;
+
+// a wave-uniform pointer moved to scalar registers (every active lane holds it)
+template <typename P> __device__ __forceinline__ P *sgpr_ptr(P *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return reinterpret_cast<P *>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+template <typename P> __device__ __forceinline__ cptr<P> uniform_ro(const P *p)
+{
+    return (cptr<P>)sgpr_ptr(p);
+}
+template <typename P> __device__ __forceinline__ gptr<P> global_ptr(P *p)
+{
+    return (gptr<P>)p;
+}
+// the configuration block of a launch (device memory, read-only while kernels run)
+__device__ __forceinline__ CfgRef cfg_of(const DevCfg *Cp) { return *uniform_ro(Cp); }
+
+template <typename T> __device__ __forceinline__ cptr<T> etab_of(CfgRef C);
+template <> __device__ __forceinline__ cptr<float> etab_of<float>(CfgRef C) { return uniform_ro(C.etabf); }
+template <> __device__ __forceinline__ cptr<double> etab_of<double>(CfgRef C) { return uniform_ro(C.etab); }
 
 // ---------------------------------------------------------------- Faddeeva
 // Dawson integral, Rybicki's method (Numerical Recipes §6.10), used only on
@@ -191,10 +229,10 @@ template <typename T> __device__ void hum4(T nu, T y, T &H, T &L)
 }
 
 // Zaghloul & Ali (2011) Algorithm 916 (series with exponential tails)
-template <typename T> __device__ void alg916(T nu, T y, const DevCfg &C, T &H, T &L)
+template <typename T> __device__ void alg916(T nu, T y, CfgRef C, T &H, T &L)
 {
     const T a = T(C.fa), aa = T(C.faa), slr = T(C.fsqrtlogr);
-    const T *E = etab_of<T>(C);
+    const auto E = etab_of<T>(C);
     const T x = mabs(nu), ya = mabs(y);
     if (x + ya > T(1e7)) {
         if (x < ya) { T r = x / ya; H = T(1) / T(kSqrtPi) / (r * r + T(1)); L = r / T(kSqrtPi) / (r * r + T(1)); }
@@ -260,7 +298,7 @@ template <typename T> __device__ void alg916(T nu, T y, const DevCfg &C, T &H, T
 // the FP32 path evaluates one rational approximation everywhere but counts
 // its calls by the region the reference would have used (the op model prices
 // each region differently).
-template <typename T> __device__ __forceinline__ int faddeeva_branch(T nu, T y, const DevCfg &C)
+template <typename T> __device__ __forceinline__ int faddeeva_branch(T nu, T y, CfgRef C)
 {
     if (nu == T(0)) return FNU0;
     if (y == T(0)) return FY0;
@@ -282,7 +320,7 @@ template <typename T> __device__ __forceinline__ int faddeeva_branch(T nu, T y, 
 }
 
 // w(z) by region of |z|^2 and accuracy nfigures; returns the branch slot
-template <typename T> __device__ __forceinline__ int faddeeva(T nu, T y, const DevCfg &C, T &H, T &L)
+template <typename T> __device__ __forceinline__ int faddeeva(T nu, T y, CfgRef C, T &H, T &L)
 {
     const T SP = T(kSqrtPi);
     if (nu == T(0)) { H = erfcx(y); L = 0; return FNU0; }
@@ -363,7 +401,7 @@ template <typename T> __device__ __forceinline__ int faddeeva(T nu, T y, const D
 // FP32 over the line-profile domain, the accuracy nfigures = 6 asks of the
 // region-split evaluation the FP64 path uses; one branch-free sequence per
 // call instead of a per-lane choice among up to nine.
-__device__ __forceinline__ void weideman16(float nu, float y, const DevCfg &C, float &H, float &L)
+__device__ __forceinline__ void weideman16(float nu, float y, CfgRef C, float &H, float &L)
 {
     const float Lw = C.wL;
     const float dr = Lw + y;                           // L - iz = dr - i nu
@@ -400,7 +438,7 @@ template <typename T> struct PixK {
     T dw, damp, eta0, hEta, S, beta, s0, s1, bnorm, lnorm, st, ct, st2, ct2p1, c2p, s2p, kQ, kU, kT, kV, shB, shV, vterm;
 };
 
-template <typename T> __device__ __forceinline__ void pix_consts(const DevCfg &C, const double *par, PixK<T> &K)
+template <typename T> __device__ __forceinline__ void pix_consts(CfgRef C, const double *par, PixK<T> &K)
 {
     K.dw = T(par[4]);
     K.damp = T(par[5]);
@@ -434,9 +472,9 @@ template <typename T> __device__ __forceinline__ void pix_consts(const DevCfg &C
 
 // the wavelength table entry a precision uses (lambda, or lambda - lambda0
 // for FP32) and the reduced wavelength formed from it
-template <typename T> __device__ __forceinline__ T wavelength_entry(const DevCfg &C, int iw);
-template <> __device__ __forceinline__ double wavelength_entry<double>(const DevCfg &C, int iw) { return C.lam[iw]; }
-template <> __device__ __forceinline__ float wavelength_entry<float>(const DevCfg &C, int iw) { return C.dlamf[iw]; }
+template <typename T> __device__ __forceinline__ T wavelength_entry(CfgRef C, int iw);
+template <> __device__ __forceinline__ double wavelength_entry<double>(CfgRef C, int iw) { return uniform_ro(C.lam)[iw]; }
+template <> __device__ __forceinline__ float wavelength_entry<float>(CfgRef C, int iw) { return uniform_ro(C.dlamf)[iw]; }
 template <typename T> __device__ __forceinline__ T reduced_wavelength_of(const PixK<T> &K, T entry);
 template <> __device__ __forceinline__ double reduced_wavelength_of<double>(const PixK<double> &K, double entry)
 {
@@ -460,7 +498,7 @@ struct NoSink {
 // me_point_at takes the wavelength table entry of iw already loaded (the
 // passes load it one wavelength ahead); me_point loads it itself
 template <typename T, bool JAC, typename Sink>
-__device__ __forceinline__ void me_point_at(const DevCfg &C, const PixK<T> &K, T step7, int iw, T lam_entry,
+__device__ __forceinline__ void me_point_at(CfgRef C, const PixK<T> &K, T step7, int iw, T lam_entry,
                                             T out[4], Sink &&sink, FCount *fcnt)
 {
     const T SP = T(kSqrtPi);
@@ -617,7 +655,7 @@ __device__ __forceinline__ void me_point_at(const DevCfg &C, const PixK<T> &K, T
 }
 
 template <typename T, bool JAC, typename Sink>
-__device__ __forceinline__ void me_point(const DevCfg &C, const PixK<T> &K, T step7, int iw, T out[4], Sink &&sink,
+__device__ __forceinline__ void me_point(CfgRef C, const PixK<T> &K, T step7, int iw, T out[4], Sink &&sink,
                                          FCount *fcnt)
 {
     me_point_at<T, JAC>(C, K, step7, iw, wavelength_entry<T>(C, iw), out, static_cast<Sink &&>(sink), fcnt);
@@ -1043,7 +1081,7 @@ __device__ __forceinline__ double clampd(double x, double lo, double hi) { retur
 
 // clamp B, fold theta into [0, pi], clamp 3..8, fold phi into [0, pi)
 // (the reference never applies the theta/phi bounds)
-__device__ __forceinline__ void bounds_check(const DevCfg &C, double p[9], double l7lo, double l7hi)
+__device__ __forceinline__ void bounds_check(CfgRef C, double p[9], double l7lo, double l7hi)
 {
     const double twopi = 2.0 * kPi;
     p[0] = clampd(p[0], C.lim[0][0], C.lim[0][1]);

@@ -50,7 +50,7 @@ struct InvArgs {
 
 template <typename TIn> __device__ __forceinline__ double ld(const void *p, int64_t i)
 {
-    return (double)((const TIn *)p)[i];
+    return (double)((const __attribute__((address_space(1))) TIn *)p)[i];
 }
 
 // Init_Guess of Src/ME_SOLVER.c:445-676 on the lane's pixel (FP64).
@@ -66,7 +66,8 @@ struct Guess {
 template <typename TIn>
 __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t pix)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
+    prof = sgpr_ptr(prof);
     Guess G;
     double *g = G.g;
     double &l7lo = G.l7lo, &l7hi = G.l7hi, &step7 = G.step7, &norm0 = G.norm0, &normp = G.normp;
@@ -113,7 +114,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
     Bpos = Bpos < 2200 ? Bpos : 2200;
     double dl;
     if (LV > thV) {
-        dl = 0.5 * (C.lam[iVmin] + C.lam[iVmax]) - lam0;
+        dl = 0.5 * (uniform_ro(C.lam)[iVmin] + uniform_ro(C.lam)[iVmax]) - lam0;
     } else {
         bool ok = false;
         if (Imin < 0.9 * Imax && iL >= C.ncut && iL < nl - C.ncut) {
@@ -124,7 +125,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
             double Im1 = ld<TIn>(prof, base + iL - 1), I0 = ld<TIn>(prof, base + iL);
             double Ip1 = ld<TIn>(prof, base + iL + 1), Ip2 = ld<TIn>(prof, base + iL + 2);
             if (Im1 < Im2 && I0 < Im1 && I0 < Ip1 && Ip1 < Ip2) {
-                dl = C.lam[iL] - lam0;
+                dl = uniform_ro(C.lam)[iL] - lam0;
                 ok = true;
             }
         }
@@ -134,7 +135,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
             int b = (C.ic + 7 < nl) ? (C.ic + 7) : (nl - 1);
             for (int iw = a; iw <= b; iw++) {
                 const double d = (Imax - ld<TIn>(prof, base + iw));
-                num += C.lam[iw] * d;
+                num += uniform_ro(C.lam)[iw] * d;
                 den += d;
             }
             dl = num / den - lam0;
@@ -166,7 +167,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
     return G;
 }
 
-__device__ __forceinline__ double penalty_of(const DevCfg &C, const double p[9])
+__device__ __forceinline__ double penalty_of(CfgRef C, const double p[9])
 {
     double pen = 0.0;
 #pragma unroll
@@ -226,10 +227,12 @@ struct PassRes {
 
 template <typename T, typename TIn>
 __device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
-                                         P9 par, double step7, double ivn, double *__restrict__ mat,
-                                         double *__restrict__ gout, int64_t NS)
+                                         P9 par, double step7, double ivn, double *__restrict__ mat_,
+                                         double *__restrict__ gout_, int64_t NS)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
+    prof = sgpr_ptr(prof);
+    const gptr<double> mat = global_ptr(mat_), gout = global_ptr(gout_);
     const int nl = C.nl;
     PixK<T> K;
     pix_consts<T>(C, par.v, K);
@@ -315,10 +318,12 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
 // whole wave.
 template <typename T, typename TIn>
 __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, bool own,
-                                          int64_t base, P9 par, double step7, double ivn, double *__restrict__ mat,
-                                          double *__restrict__ gout, int64_t NS)
+                                          int64_t base, P9 par, double step7, double ivn, double *__restrict__ mat_,
+                                          double *__restrict__ gout_, int64_t NS)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
+    prof = sgpr_ptr(prof);
+    const gptr<double> mat = global_ptr(mat_), gout = global_ptr(gout_);
     const int nl = C.nl;
     const int nh = nl >= 2 ? 2 : 1;                           // halves per pixel
     const uint64_t mask = __ballot(own);
@@ -424,7 +429,8 @@ template <typename T, typename TIn>
 __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
                                            P9 pt, double ivn)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
+    prof = sgpr_ptr(prof);
     const int nl = C.nl;
     PixK<T> K;
     pix_consts<T>(C, pt.v, K);
@@ -465,8 +471,8 @@ struct Sol9 {
     int status, nbk;
 };
 
-__device__ __forceinline__ Sol9 svd_solve_body(const double *__restrict__ in, double *__restrict__ uo,
-                                               const double *__restrict__ gv, int64_t NS, double thr0, int npar)
+__device__ __forceinline__ Sol9 svd_solve_body(const gptr<const double> in, const gptr<double> uo,
+                                               const gptr<const double> gv, int64_t NS, double thr0, int npar)
 {
     Sol9 r;
     double M[81], w[9];
@@ -543,7 +549,7 @@ __device__ __forceinline__ Sol9 svd_solve_body(const double *__restrict__ in, do
 __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
                                        const double *__restrict__ gv, int64_t NS, double thr0, int npar)
 {
-    return svd_solve_body(in, uo, gv, NS, thr0, npar);
+    return svd_solve_body(global_ptr(in), global_ptr(uo), global_ptr(gv), NS, thr0, npar);
 }
 
 // Fixed parameters (Src/LM_FIT.c:316-347): keep the rows and columns of the
@@ -551,10 +557,11 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
 // block of the lane's matrix slot (zeros elsewhere), and the gradient into its
 // first npar entries.  Each target entry reads a source at the same or a later
 // position, so the compaction runs in place in row-major order.
-__device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__restrict__ mat,
-                                       double *__restrict__ g, int64_t NS)
+__device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__restrict__ mat_,
+                                       double *__restrict__ g_, int64_t NS)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
+    const gptr<double> mat = global_ptr(mat_), g = global_ptr(g_);
     const int np = C.npar;
     for (int a = 0; a < 9; a++) {
         for (int b = 0; b < 9; b++) {
@@ -568,7 +575,7 @@ __device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__
 template <typename T, typename TIn>
 __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
 {
-    const DevCfg &C = *Cp;
+    CfgRef C = cfg_of(Cp);
     const int nl = C.nl;
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *scr = A.scratch + slot;
@@ -931,9 +938,10 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
 // the last ones running when the queue drains.  Pixels that are not
 // inverted get -1 (last).  Only the order changes, never a pixel's result.
 template <typename TIn>
-__global__ __launch_bounds__(256) void start_chi_kernel(DevCfg C, const void *prof, const double *init, int64_t n,
+__global__ __launch_bounds__(256) void start_chi_kernel(const DevCfg *__restrict__ Cp, const void *prof, const double *init, int64_t n,
                                                         float *key, int32_t *idx)
 {
+    CfgRef C = cfg_of(Cp);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     idx[p] = (int32_t)p;
@@ -973,8 +981,9 @@ __global__ __launch_bounds__(256) void start_chi_kernel(DevCfg C, const void *pr
 
 // forward synthesis, one pixel per thread
 template <typename T>
-__global__ __launch_bounds__(256) void synth_kernel(DevCfg C, const double *par, int64_t n, double *stokes, double *jac)
+__global__ __launch_bounds__(256) void synth_kernel(const DevCfg *__restrict__ Cp, const double *par, int64_t n, double *stokes, double *jac)
 {
+    CfgRef C = cfg_of(Cp);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int nl = C.nl;
@@ -1001,8 +1010,9 @@ __global__ __launch_bounds__(256) void synth_kernel(DevCfg C, const double *par,
 // best-fit profiles from the per-pixel parameters the inversion left in
 // fitpar; pixels that were not inverted (Doppler width 0) get zeros
 template <typename T>
-__global__ __launch_bounds__(256) void fit_kernel(DevCfg C, const double *fitpar, int64_t n, double *fit)
+__global__ __launch_bounds__(256) void fit_kernel(const DevCfg *__restrict__ Cp, const double *fitpar, int64_t n, double *fit)
 {
+    CfgRef C = cfg_of(Cp);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int nl = C.nl;
@@ -1037,10 +1047,11 @@ __device__ __forceinline__ float noisy_unfused(double o, double sigma, double sc
     return (float)(o + sigma * scale * g);
 }
 
-__global__ __launch_bounds__(256) void genmap_kernel(DevCfg C, uint64_t seed, const double *ranges, double ff_frac,
+__global__ __launch_bounds__(256) void genmap_kernel(const DevCfg *__restrict__ Cp, uint64_t seed, const double *ranges, double ff_frac,
                                                      double ff_bmax, double sigma, int64_t n, float *prof,
                                                      double *truth)
 {
+    CfgRef C = cfg_of(Cp);
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int nl = C.nl;
