@@ -15,6 +15,9 @@ Beside the headline (details):
     the workload's: 5 for hmi4k and niris1k, where configurations 1 and 3 put
     Philox restarts);
   * the reference's own precision (FP64 synthesis, --no-fp64 skips it);
+  * the learned run under the reference's damping bound (nine trials per LM
+    iteration instead of the learned run's two, --learned-lambda-limit 0 makes
+    it the headline): time, trials, identical-record fraction, chi2 quantiles;
   * the drop-in path from host buffers (me_invert_batch: staging + kernel +
     records back; --no-e2e skips it);
   * learned records against the FP64 oracle's learned policy fed the same
@@ -204,6 +207,9 @@ def main():
                     help="also time plain and learned at the other num_run of 1, 5, 10 (0 skips)")
     ap.add_argument("--oracle-pixels", type=int, default=16384,
                     help="pixels of the learned-vs-oracle check (0 skips it)")
+    ap.add_argument("--learned-lambda-limit", type=float, default=6e-3,
+                    help="damping bound of the learned run (6e-3: two trials per iteration; 0: lambda_limits[1], "
+                         "the reference's nine); the fallback's heuristic runs keep the reference bound")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -226,6 +232,7 @@ def main():
     over = dict(precision=ME_FP64 if fp64 else ME_FP32, block_size=args.block, waves_per_cu=args.waves_per_cu)
     over["num_run"] = args.num_run if args.num_run else w.bench_num_run
     cfg = w.config(**over)
+    over_l = dict(over, init_mode=ME_INIT_LEARNED, learned_lambda_limit=args.learned_lambda_limit)
     inv = Inversion(cfg, w.wavelengths())
     d_prof = torch.empty((n, 4, w.nl), dtype=torch.float32, device="cuda:0")
     d_res = torch.empty((n, 10), dtype=torch.float64, device="cuda:0")
@@ -320,7 +327,7 @@ def main():
     linv = None
     learned_res = None
     if args.mode in ("learned", "both"):
-        linv = Inversion(w.config(init_mode=ME_INIT_LEARNED, **over), w.wavelengths())
+        linv = Inversion(w.config(**over_l), w.wavelengths())
         train_s, loss = get_network(linv)
         learned_ms, learned_sts, learned_counted, tune = timed(linv, "learned", args.steps, args.warmup, fp64)
         learned_res = d_res.cpu().numpy()
@@ -355,12 +362,35 @@ def main():
             results["plain_fp64"]["roofline_frac"] = roofline(c64, w.nl, True, s64[-1].kernel_ms / 1e3)["frac"]
             inv64.close()
         if linv is not None:
-            l64 = Inversion(w.config(init_mode=ME_INIT_LEARNED, **o64), w.wavelengths())
+            l64 = Inversion(w.config(**dict(over_l, precision=ME_FP64)), w.wavelengths())
             l64.mlp_import(linv.mlp_params(), w.ranges())
             ms, s64, c64, _ = timed(l64, "learned fp64", 1, 0, True, tune_ok=False)
             results["learned_fp64"] = summary(ms, s64)
             results["learned_fp64"]["roofline_frac"] = roofline(c64, w.nl, True, s64[-1].kernel_ms / 1e3)["frac"]
             l64.close()
+
+    # ---- the learned run under the reference's damping bound (nine trials per
+    # iteration), same network and launch shape: what the learned-run bound
+    # changes in time, trials and records
+    if linv is not None and args.learned_lambda_limit != 0:
+        shp = results["learned"].get("launch", {})
+        e = Inversion(w.config(**dict(over_l, learned_lambda_limit=0.0, block_size=shp.get("block", 0),
+                                      waves_per_cu=shp.get("waves_per_cu", 0))), w.wavelengths())
+        e.mlp_import(linv.mlp_params(), w.ranges())
+        ms, sr, _, _ = timed(e, "learned, reference damping bound", 1, 1, fp64, tune_ok=False, count=False)
+        ref_res = d_res.cpu().numpy()
+        results["learned_reference_bound"] = summary(ms, sr)
+        results["learned_reference_bound"]["identical_records"] = round(
+            float(np.mean(np.all(ref_res == learned_res, axis=1))), 5)
+        chi_a, chi_b = learned_res[learned_res[:, 9] > 0, 9], ref_res[ref_res[:, 9] > 0, 9]
+        q = [0.1, 0.5, 0.9, 0.99]
+        results["learned_reference_bound"]["chi2_quantiles"] = {
+            "quantiles": q, "bounded": [round(float(x), 5) for x in np.quantile(chi_a, q)],
+            "reference_bound": [round(float(x), 5) for x in np.quantile(chi_b, q)]}
+        results["learned_reference_bound"]["median_error_vs_truth_converged"] = vs_truth(ref_res)
+        results["speedup_learned_bound"] = round(
+            results["learned"]["converged_pixels_per_s"] / results["learned_reference_bound"]["converged_pixels_per_s"], 3)
+        e.close()
 
     # ---- other restart budgets: num_run 1 (no Gaussian jumps), 5 and 10
     # (configuration 3's bound), plain and learned (the fallback runs num_run),
@@ -378,8 +408,7 @@ def main():
                 results[f"plain_num_run_{nr}"] = summary(ms, sr)
                 e.close()
             if linv is not None:
-                e = Inversion(w.config(**dict(over, num_run=nr, init_mode=ME_INIT_LEARNED, **shape.get("learned", {}))),
-                              w.wavelengths())
+                e = Inversion(w.config(**dict(over_l, num_run=nr, **shape.get("learned", {}))), w.wavelengths())
                 e.mlp_import(linv.mlp_params(), w.ranges())
                 ms, sr, _, _ = timed(e, f"learned num_run={nr}", 1, 0, fp64, tune_ok=False, count=False)
                 results[f"learned_num_run_{nr}"] = summary(ms, sr)
@@ -421,7 +450,7 @@ def main():
         pred = d_init.cpu().numpy()
         prof_m = d_prof[:m].cpu().numpy().astype(np.float64)
         threads = host_cpus()[0]
-        orc = O.Oracle(w.config(init_mode=ME_INIT_LEARNED, **over), w.wavelengths())
+        orc = O.Oracle(w.config(**over_l), w.wavelengths())
         ref = orc.invert(prof_m, init=pred, learned=True, nthreads=threads)["res"]
         results["learned_vs_oracle_learned_policy"] = agreement(learned_res[:m], ref, cfg.chisq_criteria)
         results["learned_vs_oracle_learned_policy"]["oracle"] = (
@@ -447,7 +476,8 @@ def main():
         "config": {"workload": f"{w.name}: Fe I {w.lambda0} A, nl={w.nl}, {w.nx}x{w.ny} px"
                                + (f", {int(w.ff_frac * 100)}% field-free" if w.ff_frac else ""),
                    "pixels": n, "init": "learned MLP + LM, fallback plain" if headline == "learned" else "heuristic",
-                   "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision},
+                   "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision,
+                   "learned_lambda_limit": args.learned_lambda_limit if headline == "learned" else None},
         "roofline": line_roof,
         "cpu_baseline": cpu,
         "details": dict(results, median_chi2=float(np.median(head_res[:, 9])),

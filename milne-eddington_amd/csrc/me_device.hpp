@@ -34,6 +34,7 @@ struct DevCfg {
     int pmap[9];              // packed index -> parameter index (ascending), first npar used
     int pidx[9];              // parameter index -> packed index, -1 for a fixed parameter
     double lambda0, bshift, icrit, crit, convg, thr, damp0, lacc, lrej, llim0, llim1, vco, lco;
+    double llim1l;            // upper damping bound of the learned run
     double step[9];
     double lim[9][2];
     double rval[9], rw[9];

@@ -802,7 +802,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 }
             } else {
                 damp *= C.lrej;
-                if (damp > C.llim1) ended = true;
+                if (damp > (lphase == 1 ? C.llim1l : C.llim1)) ended = true;
             }
         }
         // ------------------------------------------- end of an LM run (H3, H9)

@@ -6,7 +6,8 @@
 //
 //   meinv [control-file] [--precision fp64|fp32] [--batch N] [--max-chunks K]
 //         [--files CUBE.fits ... [--result-dir DIR]]
-//         [--learned [--mlp NET | --train-profiles N --train-epochs E] [--save-mlp NET]]
+//         [--learned [--mlp NET | --train-profiles N --train-epochs E] [--save-mlp NET]
+//                    [--learned-lambda-limit X]]
 //   meinv --dump-config [control-file]      parsed settings as JSON, no GPU
 //   meinv --fits-info FILE                  header summary as JSON, no GPU
 //   meinv --fits-read FILE FIRST COUNT      pixels as text, no GPU
@@ -26,7 +27,9 @@
 // the first cube on N device-synthesised profiles (default 4194304, 12
 // epochs) whose truths span the control file's bounds (theta and phi over
 // [0, pi], S over [0.5, 1.2] relative to the pixel's continuum, 30 % of the
-// pixels field-free below 50 G, noise 1e-3 S).
+// pixels field-free below 50 G, noise 1e-3 S).  --learned-lambda-limit X
+// bounds the damping of the learned run (me_config.learned_lambda_limit: the
+// library's 6e-3 allows two trials per iteration, 0 takes lambda_limits[1]).
 #include <sys/time.h>
 
 #include <algorithm>
@@ -171,6 +174,7 @@ struct Options {
     std::string mlp_in, mlp_out;
     int64_t train_profiles = 4194304;
     int train_epochs = 24;
+    double learned_limit = -1.0;  // --learned-lambda-limit; < 0 keeps the library default
 };
 
 // MLP file: magic, parameter count, the truth ranges [9][2] it was trained
@@ -447,6 +451,9 @@ int main(int argc, char **argv)
         } else if (a == "--train-epochs" && i + 1 < argc) {
             opt.train_epochs = std::atoi(argv[++i]);
             if (opt.train_epochs < 1) return fail("--train-epochs E needs E >= 1");
+        } else if (a == "--learned-lambda-limit" && i + 1 < argc) {
+            opt.learned_limit = std::atof(argv[++i]);
+            if (!(opt.learned_limit >= 0)) return fail("--learned-lambda-limit X needs X >= 0");
         } else if (a == "--result-dir" && i + 1 < argc) {
             result_dir = argv[++i];
         } else if (a == "--files") {
@@ -479,6 +486,7 @@ int main(int argc, char **argv)
     me_config cfg = ctl.cfg;
     cfg.precision = opt.prec;
     if (opt.learned) cfg.init_mode = ME_INIT_LEARNED;
+    if (opt.learned_limit >= 0) cfg.learned_lambda_limit = opt.learned_limit;
     me_ctx *ctx = nullptr;
     me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
     if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");

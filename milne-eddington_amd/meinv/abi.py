@@ -51,6 +51,7 @@ class MeConfig(C.Structure):
         ("seed", C.c_uint64),
         ("block_size", C.c_int),
         ("waves_per_cu", C.c_int),
+        ("learned_lambda_limit", C.c_double),
     ]
 
 
@@ -187,4 +188,5 @@ def reference_defaults() -> MeConfig:
     cfg.seed = 20261200
     cfg.block_size = 0
     cfg.waves_per_cu = 0
+    cfg.learned_lambda_limit = 6e-3
     return cfg

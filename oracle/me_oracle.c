@@ -1197,7 +1197,7 @@ typedef struct orc_lm_ops {
 
 /* returns the iteration count (LM_FIT's `it`); *chisq is the final chi2 before
  * the division by normp; *lm_chisq carries the last trial chi2 (H2 state) */
-static int orc_lm_control(const me_config *cf, int regul_any, const orc_lm_ops *o, double *lm_chisq,
+static int orc_lm_control(const me_config *cf, int regul_any, const orc_lm_ops *o, double llim1, double *lm_chisq,
                           double *chisq_out, int32_t *trials)
 {
     const double limit = 0.1;
@@ -1228,7 +1228,7 @@ static int orc_lm_control(const me_config *cf, int regul_any, const orc_lm_ops *
                 if (damp < cf->lambda_limits[0]) damp = cf->lambda_limits[0];
             } else {
                 damp *= cf->lambda_reject;
-                if (damp > cf->lambda_limits[1]) break;
+                if (damp > llim1) break;
             }
         } while (!accepted);
         damp = dp;
@@ -1284,13 +1284,17 @@ static void orc_op_accept(void *u)
     orc_synth_w(W, W->P, 1);
 }
 
-/* one LM run (Src/LM_FIT.c:531-661); returns the iteration count */
-static int orc_lm_fit(orc_work *W)
+/* one LM run (Src/LM_FIT.c:531-661); returns the iteration count.  The
+ * learned run (learned != 0) bounds the damping by learned_lambda_limit when
+ * that is set, every other run by lambda_limits[1]. */
+static int orc_lm_fit(orc_work *W, int learned)
 {
     const orc_lm_ops ops = {W, orc_op_start, orc_op_penalty, orc_op_normal, orc_op_trial, orc_op_accept};
+    const me_config *cf = &W->c->cfg;
+    const double llim1 = (learned && cf->learned_lambda_limit > 0) ? cf->learned_lambda_limit : cf->lambda_limits[1];
     double chisq;
     int32_t tr = 0;
-    const int it = orc_lm_control(&W->c->cfg, W->c->regul_any, &ops, &W->lm_chisq, &chisq, &tr);
+    const int it = orc_lm_control(cf, W->c->regul_any, &ops, llim1, &W->lm_chisq, &chisq, &tr);
     W->trials += tr;
     W->iters += it;
     W->chisq = chisq / W->normp;
@@ -1331,7 +1335,7 @@ int orc_lm_replay(const me_config *cfg, int regul_any, double chi0, double pen0,
     orc_replay R = {chi0, pen0, trial_chi, trial_pen, ntrial, 0, ratios, nratio_max, 0};
     const orc_lm_ops ops = {&R, orc_rp_start, orc_rp_penalty, orc_rp_normal, orc_rp_trial, orc_rp_accept};
     int32_t tr = 0;
-    const int it = orc_lm_control(cfg, regul_any, &ops, lm_chisq, chisq, &tr);
+    const int it = orc_lm_control(cfg, regul_any, &ops, cfg->lambda_limits[1], lm_chisq, chisq, &tr);
     *trials = tr;
     *nratios = R.nr;
     return it;
@@ -1441,7 +1445,7 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
     if (learned) {
         for (int k = 0; k < 9; k++) W->P[k] = init[k];
         orc_bounds_check(W->P, (const double(*)[2])W->lim);
-        orc_lm_fit(W);
+        orc_lm_fit(W, 1);
         (*runs)++;
         if (W->chisq <= cf->chisq_criteria || cf->num_run == 0) {
             orc_sav(W);
@@ -1459,7 +1463,7 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
     int irun;
     for (irun = 0; irun < cf->num_run * 4; irun++) {
         if (irun > 0) orc_jump(W, key, irun);
-        orc_lm_fit(W);
+        orc_lm_fit(W, 0);
         (*runs)++;
         int save, brk;
         orc_policy(irun, cf->num_run, cf->chisq_criteria, W->chisq, W->chisq_best, &save, &brk);

@@ -80,3 +80,65 @@ def test_learned_fp32_error_against_the_oracle(learned_case):
           f"{np.median(d[:, 3]) * 1e3:.3g} m/s")
     assert abs(conv_a - conv_b) <= 0.02
     assert phys >= 0.6
+
+
+# Learned run bounded to two trials per LM iteration (learned_lambda_limit:
+# 1e-3 * 5^2 > 6e-3 ends an iteration at its second rejected trial); the
+# heuristic runs of the fallback keep lambda_limits[1]
+LEARNED_LIMIT = 6e-3
+
+
+def test_learned_two_trial_bound_matches_the_oracle(learned_case):
+    """FP64 engine and the oracle's learned policy under the same learned-run
+    bound, seeded by the same predictions: the learned run never solves with a
+    rounding-decided matrix, so its records agree on (nearly) every pixel that
+    does not fall back; fallback pixels agree at the chaos floor."""
+    w, params, prof, pred, _ = learned_case
+    prof64 = prof.astype(np.float64)
+    cfg = w.config(init_mode=ME_INIT_LEARNED, learned_lambda_limit=LEARNED_LIMIT)
+    orc = O.Oracle(cfg, w.wavelengths())
+    ref = orc.invert(prof64, init=pred, learned=True, nthreads=8)
+    with gpu_inversion("hmi4k", init_mode=ME_INIT_LEARNED, learned_lambda_limit=LEARNED_LIMIT) as inv:
+        out = inv.invert(prof64, init=pred, per_pixel=True)
+    pp = out["stats"].per_pixel
+    ok = within_parity(out["res"], ref["res"])
+    fb_dev = (pp["status"] & 4) != 0
+    fb_orc = (ref["status"] & 4) != 0
+    direct = ~fb_dev & ~fb_orc
+    print(f"learned FP64 engine vs oracle, two-trial learned run: {ok.mean():.4f} within tolerance, "
+          f"{ok[direct].mean():.4f} of pixels without fallback; fallback {fb_dev.mean():.4f} / {fb_orc.mean():.4f}")
+    assert np.mean(fb_dev == fb_orc) >= 0.995
+    assert ok[direct].mean() >= 0.99
+    # a pixel without fallback ran one LM run of at most two trials per
+    # iteration: its iterations that accepted a trial, plus the last one
+    assert np.all(pp["trials"][direct] <= 2 * (pp["iters"][direct] + 1))
+    assert np.all(ref["trials"][direct] <= 2 * (ref["iters"][direct] + 1))
+
+
+def test_learned_two_trial_bound_keeps_the_records(learned_case):
+    """FP32 engine, headline form: the bound changes the learned run's trial
+    count, not its outcome; converged fraction and chi2 distribution against
+    the reference damping bound, on the device's own predictions."""
+    import torch
+    w, params, prof, pred, _ = learned_case
+    d_prof = torch.from_numpy(prof).to("cuda:0")
+    d_res = device_tensor((N, 10), torch.float64)
+    recs, trials = [], []
+    for lim in (0.0, LEARNED_LIMIT):
+        with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED, learned_lambda_limit=lim) as inv:
+            inv.mlp_import(params, w.ranges())
+            st = inv.invert_device(d_prof.data_ptr(), N, d_res.data_ptr())
+            recs.append(d_res.cpu().numpy())
+            trials.append(st.n_trial)
+    a, b = recs
+    ca = (a[:, 9] > 0) & (a[:, 9] <= w.chisq_criteria)
+    cb = (b[:, 9] > 0) & (b[:, 9] <= w.chisq_criteria)
+    same = np.mean(np.all(a == b, axis=1))
+    qa = np.quantile(a[a[:, 9] > 0, 9], [0.5, 0.9, 0.99])
+    qb = np.quantile(b[b[:, 9] > 0, 9], [0.5, 0.9, 0.99])
+    print(f"two-trial learned run: trials {trials[1]} vs {trials[0]}, converged {cb.mean():.4f} vs {ca.mean():.4f}, "
+          f"identical records {same:.4f}, chi2 quantiles {qb} vs {qa}")
+    assert trials[1] < trials[0]
+    assert cb.mean() >= ca.mean() - 0.002
+    assert same >= 0.99
+    assert np.allclose(qa, qb, rtol=0.02)

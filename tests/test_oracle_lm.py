@@ -257,3 +257,47 @@ def test_learned_policy_one_run_from_a_good_prediction():
     np.testing.assert_array_equal(lrn["res"][fb][took_plain], plain["res"][fb][took_plain])
     # without fallback the record is the learned run's
     assert np.all(lrn["runs"][~fb] == 1)
+
+
+def test_learned_run_damping_bound_applies_to_the_learned_run_only():
+    """learned_lambda_limit = 6e-3 ends an iteration of the learned run at its
+    second rejected trial (1e-3 * 5^2 > 6e-3 >= 1e-3 * 5), so a pixel without
+    fallback runs at most 2 trials per iteration it starts; the heuristic runs
+    of a fallback keep lambda_limits[1] (9 rejected trials end an iteration)."""
+    w = get("hmi4k")
+    prof, truth = O.Oracle(w.config(), w.wavelengths()).gen_map(w.seed, w.ranges(), 200, ff_frac=w.ff_frac,
+                                                                 ff_bmax=w.ff_bmax, sigma=w.sigma)
+    prof = prof.astype(np.float64)
+    start = truth.copy()
+    start[:, 0] *= 1.3          # a start near, not at, the truth
+    start[:, 3] += 0.3
+    ref = O.Oracle(w.config(num_run=1, learned_lambda_limit=0.0), w.wavelengths()).invert(
+        prof, init=start, learned=True, nthreads=8)
+    cap = O.Oracle(w.config(num_run=1, learned_lambda_limit=6e-3), w.wavelengths()).invert(
+        prof, init=start, learned=True, nthreads=8)
+    direct = ((ref["status"] & 4) == 0) & ((cap["status"] & 4) == 0) & (ref["runs"] > 0)
+    assert direct.mean() > 0.5
+    assert np.all(cap["trials"][direct] <= 2 * (cap["iters"][direct] + 1))
+    # under the reference bound a learned run ends with 9 rejected trials
+    # unless it converges on the relative-gain test, so more trials run
+    assert np.any(ref["trials"][direct] > 2 * (ref["iters"][direct] + 1))
+    assert cap["trials"].sum() < ref["trials"].sum()
+    # from a poor start most pixels fall back: their heuristic runs are the
+    # plain inversion's (same Philox keys), with the reference bound, so
+    # subtracting the plain inversion's counts leaves the learned run's own
+    bad = truth.copy()
+    bad[:, 6], bad[:, 4], bad[:, 0] = 1.0, 80.0, 3000.0
+    plain = O.Oracle(w.config(num_run=1), w.wavelengths()).invert(prof, nthreads=8)
+    for lim, bounded in ((0.0, False), (6e-3, True)):
+        r = O.Oracle(w.config(num_run=1, learned_lambda_limit=lim), w.wavelengths()).invert(
+            prof, init=bad, learned=True, nthreads=8)
+        fb = (r["status"] & 4) != 0
+        assert fb.mean() > 0.3
+        assert np.all(r["runs"][fb] == plain["runs"][fb] + 1)
+        lt = r["trials"][fb] - plain["trials"][fb]
+        li = r["iters"][fb] - plain["iters"][fb]
+        assert np.all(lt >= 1) and np.all(li >= 0)
+        if bounded:
+            assert np.all(lt <= 2 * (li + 1))
+        else:
+            assert np.any(lt > 2 * (li + 1))
