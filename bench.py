@@ -15,9 +15,11 @@ Beside the headline (details):
     the workload's: 5 for hmi4k and niris1k, where configurations 1 and 3 put
     Philox restarts);
   * the reference's own precision (FP64 synthesis, --no-fp64 skips it);
-  * the learned run under the reference's damping bound (nine trials per LM
-    iteration instead of the learned run's two, --learned-lambda-limit 0 makes
-    it the headline): time, trials, identical-record fraction, chi2 quantiles;
+  * the learned run under the reference's LM settings (nine trials per LM
+    iteration instead of the learned run's two, S bounded to 1.4 Imax instead
+    of 2 Imax; --learned-lambda-limit 0 --learned-s-limit 0 make them the
+    headline): time, trials, converged fraction, identical records, chi2
+    quantiles;
   * the drop-in path from host buffers (me_invert_batch: staging + kernel +
     records back; --no-e2e skips it);
   * learned records against the FP64 oracle's learned policy fed the same
@@ -210,6 +212,11 @@ def main():
     ap.add_argument("--learned-lambda-limit", type=float, default=6e-3,
                     help="damping bound of the learned run (6e-3: two trials per iteration; 0: lambda_limits[1], "
                          "the reference's nine); the fallback's heuristic runs keep the reference bound")
+    ap.add_argument("--learned-s-limit", type=float, default=2.0,
+                    help="upper bound of S in the learned run in units of the profile's I maximum (0: "
+                         "Init_Guess's 1.4, which every heuristic run keeps)")
+    ap.add_argument("--no-ref-bound", action="store_true",
+                    help="skip the learned run under the reference's LM settings")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -232,7 +239,8 @@ def main():
     over = dict(precision=ME_FP64 if fp64 else ME_FP32, block_size=args.block, waves_per_cu=args.waves_per_cu)
     over["num_run"] = args.num_run if args.num_run else w.bench_num_run
     cfg = w.config(**over)
-    over_l = dict(over, init_mode=ME_INIT_LEARNED, learned_lambda_limit=args.learned_lambda_limit)
+    over_l = dict(over, init_mode=ME_INIT_LEARNED, learned_lambda_limit=args.learned_lambda_limit,
+                  learned_s_limit=args.learned_s_limit)
     inv = Inversion(cfg, w.wavelengths())
     d_prof = torch.empty((n, 4, w.nl), dtype=torch.float32, device="cuda:0")
     d_res = torch.empty((n, 10), dtype=torch.float64, device="cuda:0")
@@ -369,27 +377,29 @@ def main():
             results["learned_fp64"]["roofline_frac"] = roofline(c64, w.nl, True, s64[-1].kernel_ms / 1e3)["frac"]
             l64.close()
 
-    # ---- the learned run under the reference's damping bound (nine trials per
-    # iteration), same network and launch shape: what the learned-run bound
-    # changes in time, trials and records
-    if linv is not None and args.learned_lambda_limit != 0:
+    # ---- the learned run under the reference's LM settings (nine trials per
+    # iteration, S bounded to 1.4 Imax as Init_Guess does), same network and
+    # launch shape: what the learned run's own bounds change in time, trials,
+    # converged fraction and records
+    if linv is not None and (args.learned_lambda_limit != 0 or args.learned_s_limit != 0) and not args.no_ref_bound:
         shp = results["learned"].get("launch", {})
-        e = Inversion(w.config(**dict(over_l, learned_lambda_limit=0.0, block_size=shp.get("block", 0),
-                                      waves_per_cu=shp.get("waves_per_cu", 0))), w.wavelengths())
+        e = Inversion(w.config(**dict(over_l, learned_lambda_limit=0.0, learned_s_limit=0.0,
+                                      block_size=shp.get("block", 0), waves_per_cu=shp.get("waves_per_cu", 0))),
+                      w.wavelengths())
         e.mlp_import(linv.mlp_params(), w.ranges())
-        ms, sr, _, _ = timed(e, "learned, reference damping bound", 1, 1, fp64, tune_ok=False, count=False)
+        ms, sr, _, _ = timed(e, "learned, reference LM settings", 1, 1, fp64, tune_ok=False, count=False)
         ref_res = d_res.cpu().numpy()
-        results["learned_reference_bound"] = summary(ms, sr)
-        results["learned_reference_bound"]["identical_records"] = round(
-            float(np.mean(np.all(ref_res == learned_res, axis=1))), 5)
+        key = "learned_reference_settings"
+        results[key] = summary(ms, sr)
+        results[key]["identical_records"] = round(float(np.mean(np.all(ref_res == learned_res, axis=1))), 5)
         chi_a, chi_b = learned_res[learned_res[:, 9] > 0, 9], ref_res[ref_res[:, 9] > 0, 9]
         q = [0.1, 0.5, 0.9, 0.99]
-        results["learned_reference_bound"]["chi2_quantiles"] = {
-            "quantiles": q, "bounded": [round(float(x), 5) for x in np.quantile(chi_a, q)],
-            "reference_bound": [round(float(x), 5) for x in np.quantile(chi_b, q)]}
-        results["learned_reference_bound"]["median_error_vs_truth_converged"] = vs_truth(ref_res)
-        results["speedup_learned_bound"] = round(
-            results["learned"]["converged_pixels_per_s"] / results["learned_reference_bound"]["converged_pixels_per_s"], 3)
+        results[key]["chi2_quantiles"] = {
+            "quantiles": q, "learned_bounds": [round(float(x), 5) for x in np.quantile(chi_a, q)],
+            "reference_settings": [round(float(x), 5) for x in np.quantile(chi_b, q)]}
+        results[key]["median_error_vs_truth_converged"] = vs_truth(ref_res)
+        results["speedup_learned_bounds"] = round(
+            results["learned"]["converged_pixels_per_s"] / results[key]["converged_pixels_per_s"], 3)
         e.close()
 
     # ---- other restart budgets: num_run 1 (no Gaussian jumps), 5 and 10
@@ -477,7 +487,8 @@ def main():
                                + (f", {int(w.ff_frac * 100)}% field-free" if w.ff_frac else ""),
                    "pixels": n, "init": "learned MLP + LM, fallback plain" if headline == "learned" else "heuristic",
                    "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision,
-                   "learned_lambda_limit": args.learned_lambda_limit if headline == "learned" else None},
+                   "learned_lambda_limit": args.learned_lambda_limit if headline == "learned" else None,
+                   "learned_s_limit": args.learned_s_limit if headline == "learned" else None},
         "roofline": line_roof,
         "cpu_baseline": cpu,
         "details": dict(results, median_chi2=float(np.median(head_res[:, 9])),

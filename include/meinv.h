@@ -89,6 +89,14 @@ typedef struct me_config {
      * an orthogonal matrix, which rounding decides).  The heuristic runs of
      * the fallback always use lambda_limits[1]. */
     double   learned_lambda_limit;
+    /* upper bound of S in the learned run, in units of the profile's I
+     * maximum; 2 by default, 0 keeps Init_Guess's 1.4.  Init_Guess starts S at
+     * Imax and bounds it to [0.6, 1.4] Imax (Src/ME_SOLVER.c:662-666), so a
+     * line that never reaches the continuum inside the wavelength window
+     * (S above 1.4 Imax) cannot be fitted; the learned run starts from the
+     * network's S and may go up to this bound.  Every heuristic run, the
+     * fallback's included, keeps [0.6, 1.4] Imax. */
+    double   learned_s_limit;
 } me_config;
 
 /* Aggregate counters and optional per-pixel arrays filled by me_invert_*.

@@ -500,6 +500,7 @@ void me_config_defaults(me_config *c)
     c->init_mode = ME_INIT_HEURISTIC;
     c->seed = 20261200;
     c->learned_lambda_limit = 6e-3;
+    c->learned_s_limit = 2.0;
 }
 
 me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_ctx **out)
@@ -514,6 +515,8 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     // repeat forever (the reference loops there)
     if (!(cfg->damp > 0) || !std::isfinite(cfg->lambda_limits[1]) || !std::isfinite(cfg->damp)) return ME_EINVAL;
     if (!(cfg->learned_lambda_limit >= 0) || !std::isfinite(cfg->learned_lambda_limit)) return ME_EINVAL;
+    if (!(cfg->learned_s_limit == 0 || cfg->learned_s_limit >= 0.6) || !std::isfinite(cfg->learned_s_limit))
+        return ME_EINVAL;
     if (cfg->block_size != 0 && (cfg->block_size < 64 || cfg->block_size > 256 || cfg->block_size % 64))
         return ME_EINVAL;  // the kernel's LDS counters hold 256 lanes
     if (cfg->precision != ME_FP64 && cfg->precision != ME_FP32) return ME_EINVAL;
@@ -586,6 +589,7 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     d.llim0 = cfg->lambda_limits[0];
     d.llim1 = cfg->lambda_limits[1];
     d.llim1l = cfg->learned_lambda_limit > 0 ? cfg->learned_lambda_limit : cfg->lambda_limits[1];
+    d.ls7 = cfg->learned_s_limit;
     d.vco = cfg->vcoeffi;
     d.lco = cfg->lcoeffi;
     for (int s = 0; s < 4; s++) d.wsq[s] = cfg->weights[s] * cfg->weights[s];

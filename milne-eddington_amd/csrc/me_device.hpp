@@ -35,6 +35,7 @@ struct DevCfg {
     int pidx[9];              // parameter index -> packed index, -1 for a fixed parameter
     double lambda0, bshift, icrit, crit, convg, thr, damp0, lacc, lrej, llim0, llim1, vco, lco;
     double llim1l;            // upper damping bound of the learned run
+    double ls7;               // learned run: S bound in units of Imax (0: Init_Guess's)
     double step[9];
     double lim[9][2];
     double rval[9], rw[9];

@@ -655,6 +655,9 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 par[k] = gs[k];
             }
             if (A.learned) {
+                // the learned run may bound S wider than Init_Guess (gs[7] is
+                // Imax); the heuristic runs of a fallback restore its bound
+                if (C.ls7 > 0) l7hi = C.ls7 * gs[7];
 #pragma unroll
                 for (int k = 0; k < 9; k++) par[k] = ini[k];
                 bounds_check(C, par, l7lo, l7hi);
@@ -829,6 +832,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                         par[k] = SCR(SL_GUESS + k);
                     }
                     SCR(SL_CHI_LEARN) = crun;
+                    l7hi = SCR(SL_GUESS + 7) * 1.4;   // Init_Guess's bound (Imax * 1.4)
                     lphase = 2;
                     px_status |= 4;
                     CT(CT_FALLBACK) += 1;

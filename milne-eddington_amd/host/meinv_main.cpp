@@ -7,7 +7,7 @@
 //   meinv [control-file] [--precision fp64|fp32] [--batch N] [--max-chunks K]
 //         [--files CUBE.fits ... [--result-dir DIR]]
 //         [--learned [--mlp NET | --train-profiles N --train-epochs E] [--save-mlp NET]
-//                    [--learned-lambda-limit X]]
+//                    [--learned-lambda-limit X] [--learned-s-limit F]]
 //   meinv --dump-config [control-file]      parsed settings as JSON, no GPU
 //   meinv --fits-info FILE                  header summary as JSON, no GPU
 //   meinv --fits-read FILE FIRST COUNT      pixels as text, no GPU
@@ -29,7 +29,9 @@
 // [0, pi], S over [0.5, 1.2] relative to the pixel's continuum, 30 % of the
 // pixels field-free below 50 G, noise 1e-3 S).  --learned-lambda-limit X
 // bounds the damping of the learned run (me_config.learned_lambda_limit: the
-// library's 6e-3 allows two trials per iteration, 0 takes lambda_limits[1]).
+// library's 6e-3 allows two trials per iteration, 0 takes lambda_limits[1]);
+// --learned-s-limit F bounds S in the learned run to F times the profile's I
+// maximum (me_config.learned_s_limit: 2 by default, 0 keeps Init_Guess's 1.4).
 #include <sys/time.h>
 
 #include <algorithm>
@@ -175,6 +177,7 @@ struct Options {
     int64_t train_profiles = 4194304;
     int train_epochs = 24;
     double learned_limit = -1.0;  // --learned-lambda-limit; < 0 keeps the library default
+    double learned_s = -1.0;      // --learned-s-limit; < 0 keeps the library default
 };
 
 // MLP file: magic, parameter count, the truth ranges [9][2] it was trained
@@ -454,6 +457,9 @@ int main(int argc, char **argv)
         } else if (a == "--learned-lambda-limit" && i + 1 < argc) {
             opt.learned_limit = std::atof(argv[++i]);
             if (!(opt.learned_limit >= 0)) return fail("--learned-lambda-limit X needs X >= 0");
+        } else if (a == "--learned-s-limit" && i + 1 < argc) {
+            opt.learned_s = std::atof(argv[++i]);
+            if (!(opt.learned_s == 0 || opt.learned_s >= 0.6)) return fail("--learned-s-limit F needs F = 0 or F >= 0.6");
         } else if (a == "--result-dir" && i + 1 < argc) {
             result_dir = argv[++i];
         } else if (a == "--files") {
@@ -487,6 +493,7 @@ int main(int argc, char **argv)
     cfg.precision = opt.prec;
     if (opt.learned) cfg.init_mode = ME_INIT_LEARNED;
     if (opt.learned_limit >= 0) cfg.learned_lambda_limit = opt.learned_limit;
+    if (opt.learned_s >= 0) cfg.learned_s_limit = opt.learned_s;
     me_ctx *ctx = nullptr;
     me_status st = me_ctx_create(&cfg, lambda.data(), nl, &ctx);
     if (st != ME_OK) return fail("me_ctx_create failed (status " + std::to_string((int)st) + ")");

@@ -52,6 +52,7 @@ class MeConfig(C.Structure):
         ("block_size", C.c_int),
         ("waves_per_cu", C.c_int),
         ("learned_lambda_limit", C.c_double),
+        ("learned_s_limit", C.c_double),
     ]
 
 
@@ -189,4 +190,5 @@ def reference_defaults() -> MeConfig:
     cfg.block_size = 0
     cfg.waves_per_cu = 0
     cfg.learned_lambda_limit = 6e-3
+    cfg.learned_s_limit = 2.0
     return cfg

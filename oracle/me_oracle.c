@@ -1443,6 +1443,8 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
 
     double PL[9], chiL = 0.0;
     if (learned) {
+        /* the learned run's S bound (learned_s_limit x Imax; Pg[7] is Imax) */
+        if (cf->learned_s_limit > 0) W->lim[7][1] = cf->learned_s_limit * W->Pg[7];
         for (int k = 0; k < 9; k++) W->P[k] = init[k];
         orc_bounds_check(W->P, (const double(*)[2])W->lim);
         orc_lm_fit(W, 1);
@@ -1452,6 +1454,7 @@ static int orc_inversion(orc_work *W, int64_t pix, int64_t n, const double *init
             goto done;
         }
         *fallback = 1;
+        W->lim[7][1] = lim7[1];
         for (int k = 0; k < 9; k++) {
             PL[k] = W->P[k];
             W->P[k] = W->Pg[k];

@@ -38,17 +38,20 @@ def learned_case():
     return w, params, prof, pred, res32_net
 
 
-def test_learned_fp64_engine_matches_the_oracle(learned_case):
+@pytest.mark.parametrize("s_limit", [0.0, 2.0])
+def test_learned_fp64_engine_matches_the_oracle(learned_case, s_limit):
     """FP64 engine in learned mode seeded by the predictions vs the oracle's
-    learned policy seeded by the same values: agreement at the chaos floor."""
+    learned policy seeded by the same values: agreement at the chaos floor,
+    with the learned run's S bound at Init_Guess's 1.4 Imax (0) and at the
+    default 2 Imax."""
     w, params, prof, pred, _ = learned_case
     prof64 = prof.astype(np.float64)
-    orc = O.Oracle(w.config(init_mode=ME_INIT_LEARNED), w.wavelengths())
+    orc = O.Oracle(w.config(init_mode=ME_INIT_LEARNED, learned_s_limit=s_limit), w.wavelengths())
     ref = orc.invert(prof64, init=pred, learned=True, nthreads=8)
     rng = np.random.default_rng(0)
     self_ = orc.invert(prof64 * (1 + 1e-15 * rng.standard_normal(prof64.shape)), init=pred, learned=True, nthreads=8)
     floor = within_parity(self_["res"], ref["res"]).mean()
-    with gpu_inversion("hmi4k", init_mode=ME_INIT_LEARNED) as inv:
+    with gpu_inversion("hmi4k", init_mode=ME_INIT_LEARNED, learned_s_limit=s_limit) as inv:
         out = inv.invert(prof64, init=pred, per_pixel=True)
     ok = within_parity(out["res"], ref["res"])
     fb_dev = (out["stats"].per_pixel["status"] & 4) != 0
@@ -134,8 +137,10 @@ def test_learned_two_trial_bound_keeps_the_records(learned_case):
     ca = (a[:, 9] > 0) & (a[:, 9] <= w.chisq_criteria)
     cb = (b[:, 9] > 0) & (b[:, 9] <= w.chisq_criteria)
     same = np.mean(np.all(a == b, axis=1))
-    qa = np.quantile(a[a[:, 9] > 0, 9], [0.5, 0.9, 0.99])
-    qb = np.quantile(b[b[:, 9] > 0, 9], [0.5, 0.9, 0.99])
+    # the chi2 distribution below its tail (the top percent is the pixels
+    # that converge nowhere, whose last record depends on which run ends last)
+    qa = np.quantile(a[a[:, 9] > 0, 9], [0.1, 0.5, 0.9])
+    qb = np.quantile(b[b[:, 9] > 0, 9], [0.1, 0.5, 0.9])
     print(f"two-trial learned run: trials {trials[1]} vs {trials[0]}, converged {cb.mean():.4f} vs {ca.mean():.4f}, "
           f"identical records {same:.4f}, chi2 quantiles {qb} vs {qa}")
     assert trials[1] < trials[0]

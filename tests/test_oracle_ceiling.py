@@ -6,7 +6,9 @@ Starting each pixel's LM run at its truth (the oracle's learned policy fed
 the truths) with num_run = 1, the converged fraction stays below 99 %, and
 every pixel that fails to converge is one of those: without Gaussian restarts
 no run leaves the bounded-S basin.  (Restarts, num_run >= 5, find other fits
-under the criterion for some of them.)"""
+under the criterion for some of them.)  The learned run's own S bound of
+2 Imax (learned_s_limit) lifts the ceiling: from the truth every pixel
+converges."""
 import numpy as np
 
 import oracle as O
@@ -15,7 +17,7 @@ from meinv.workloads import get
 
 def test_starting_at_the_truth_converges_below_99_percent():
     w = get("hmi4k")
-    orc = O.Oracle(w.config(), w.wavelengths())
+    orc = O.Oracle(w.config(learned_s_limit=0.0), w.wavelengths())
     n = 8192
     prof, truth = orc.gen_map(w.seed, w.ranges(), n, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
     p64 = prof.astype(np.float64)
@@ -28,3 +30,9 @@ def test_starting_at_the_truth_converges_below_99_percent():
     assert 0.98 <= conv.mean() < 0.99
     assert not np.any(~conv & ~s_bound)
     assert s_bound.mean() > 0.03
+    # the learned run's default bound, 2 Imax
+    orc2 = O.Oracle(w.config(learned_s_limit=2.0), w.wavelengths())
+    res2 = orc2.invert(p64, init=truth, learned=True, nthreads=8)["res"]
+    conv2 = (res2[:, 9] > 0) & (res2[:, 9] <= w.chisq_criteria)
+    print(f"from the truth, S up to 2 Imax: converged {conv2.mean():.4f}")
+    assert conv2.mean() >= 0.999

@@ -301,3 +301,38 @@ def test_learned_run_damping_bound_applies_to_the_learned_run_only():
             assert np.all(lt <= 2 * (li + 1))
         else:
             assert np.any(lt > 2 * (li + 1))
+
+
+def test_learned_run_s_bound_reaches_lines_without_continuum():
+    """Init_Guess bounds S to [0.6, 1.4] Imax (Src/ME_SOLVER.c:662-666): a line
+    that never reaches the continuum in the window (truth S above 1.4 Imax)
+    cannot be fitted by any heuristic run.  learned_s_limit = 2 lets the
+    learned run go up to 2 Imax: started at the truth it converges there, with
+    S above 1.4 Imax and no fallback; with 0 (Init_Guess's bound) the same
+    starts are clamped and fall back.  Heuristic inversions never see it."""
+    w = get("hmi4k")
+    orc0 = O.Oracle(w.config(num_run=1, learned_s_limit=0.0), w.wavelengths())
+    orc2 = O.Oracle(w.config(num_run=1, learned_s_limit=2.0), w.wavelengths())
+    prof, truth = orc0.gen_map(w.seed, w.ranges(), 3000, ff_frac=w.ff_frac, ff_bmax=w.ff_bmax, sigma=w.sigma)
+    prof = prof.astype(np.float64)
+    imax = prof[:, 0, :].max(axis=1)
+    hi = truth[:, 7] > 1.45 * imax
+    assert 40 <= hi.sum() <= 300
+    p, t, im = prof[hi], truth[hi], imax[hi]
+    a = orc0.invert(p, init=t, learned=True, nthreads=8)
+    b = orc2.invert(p, init=t, learned=True, nthreads=8)
+    crit = w.chisq_criteria
+    # Init_Guess's bound holds for every run of a: every record has S <= 1.4
+    # Imax; a third of these pixels miss the criterion from the clamped
+    # truth and fall back (at nl = 6 the others fit within the bound)
+    assert np.all(a["res"][:, 7] <= 1.4 * im * (1 + 1e-12))
+    assert np.mean((a["status"] & 4) != 0) > 0.2
+    conv_b = (b["res"][:, 9] > 0) & (b["res"][:, 9] <= crit)
+    direct = (b["status"] & 4) == 0
+    assert np.mean(conv_b & direct) > 0.97
+    assert b["trials"].sum() < a["trials"].sum()
+    assert np.all(b["res"][direct, 7] <= 2.0 * im[direct] * (1 + 1e-12))
+    assert np.mean(b["res"][direct, 7] > 1.4 * im[direct]) > 0.9
+    # heuristic inversions: identical records whatever the learned bound
+    plain0, plain2 = orc0.invert(prof[:200], nthreads=8), orc2.invert(prof[:200], nthreads=8)
+    np.testing.assert_array_equal(plain0["res"], plain2["res"])
