@@ -196,8 +196,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--tune-pixels", type=int, default=1 << 20,
-                    help="autotune the launch shape on this many pixels first (0: keep --block/--waves-per-cu)")
+    ap.add_argument("--tune-pixels", type=int, default=1 << 22,
+                    help="autotune the launch shape on this many pixels first, about 64 per resident lane "
+                         "(0: keep --block/--waves-per-cu)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=4 * 1024 * 1024)
     ap.add_argument("--train-epochs", type=int, default=24)

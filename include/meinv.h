@@ -159,11 +159,13 @@ me_status me_ctx_set_region_counts(me_ctx *ctx, int on);
 
 /* Launch-shape autotuner of the persistent inversion kernel: inverts the
  * first n_sample device profiles under every candidate block size (64, 128,
- * 256 lanes) and resident-wave count per CU that the occupancy allows (best
- * of two runs each, the context's init mode and precision), keeps the fastest
- * in the context's block_size / waves_per_cu and reports it.  Results never
- * depend on the launch shape.  No counterpart in the reference (its MPI
- * chunk size, nprofiles, is not a kernel shape). */
+ * 256 lanes) and resident-wave count per CU that the occupancy allows and
+ * that gives each SIMD of a CU a wave (best of two runs each, the context's
+ * init mode and precision), keeps the fastest in the context's block_size /
+ * waves_per_cu and reports it.  The sample should hold some 64 pixels per
+ * resident lane (4M pixels on 256 CUs): a smaller one is timed by its slowest
+ * pixels.  Results never depend on the launch shape.  No counterpart in the
+ * reference (its MPI chunk size, nprofiles, is not a kernel shape). */
 me_status me_ctx_autotune(me_ctx *ctx, const void *d_prof, int prof_is_f64, int64_t n_sample,
                           int *block_size, int *waves_per_cu, double *kernel_ms);
 

@@ -64,6 +64,9 @@ struct me_ctx {
 
 namespace {
 
+// SIMDs per compute unit of CDNA3/CDNA4 (gfx942, gfx950)
+constexpr int kSimdsPerCU = 4;
+
 constexpr double kThr[5][5] = {{1.6e4, 160.0, 107.0, 28.5, 3.5},
                                {1.5e5, 510.0, 110.0, 39., -0.1},
                                {1.451e6, 1.6e3, 180.0, 110.0, -0.1},
@@ -809,14 +812,22 @@ me_status me_ctx_autotune(me_ctx *c, const void *d_prof, int prof_is_f64, int64_
     auto kern = c->cfg.precision == ME_FP64
                     ? (prof_is_f64 ? (const void *)me::invert_kernel<double, double> : (const void *)me::invert_kernel<double, float>)
                     : (prof_is_f64 ? (const void *)me::invert_kernel<float, double> : (const void *)me::invert_kernel<float, float>);
+    // Only shapes that give every SIMD of a CU at least one wave: a shape that
+    // leaves SIMDs idle can only win on a sample whose time is set by its
+    // slowest pixels (a tail of restarted fallback pixels running alone),
+    // and then loses by the idle fraction on a full map.  The kernel's LDS
+    // columns are sized for 256 lanes, so smaller blocks fit once per CU and
+    // drop out unless the occupancy allows them enough waves.
     struct Cand { int block, waves; };
     std::vector<Cand> cands;
     for (int block : {64, 128, 256}) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
         const int wmax = per_cu * (block / 64);
-        for (int waves = wmax; waves >= block / 64 && waves >= wmax / 2; waves -= block / 64) cands.push_back({block, waves});
+        for (int waves = wmax; waves >= block / 64 && waves >= wmax / 2; waves -= block / 64)
+            if (waves >= kSimdsPerCU) cands.push_back({block, waves});
     }
+    if (cands.empty()) cands.push_back({256, 0});
     double *d_res = nullptr;
     HIPCHK(hipMalloc(&d_res, sizeof(double) * 10 * n));
     const int keep_block = c->cfg.block_size, keep_waves = c->cfg.waves_per_cu;

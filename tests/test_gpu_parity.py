@@ -102,6 +102,24 @@ def test_inversion_fp64_parity_two_trial_cap(name):
     assert ok.mean() >= floor - 0.01
 
 
+@pytest.mark.parametrize("wexp", [-120, 120])
+@pytest.mark.parametrize("name", ["niris64", "hmi4k"])
+def test_inversion_fp64_parity_extreme_weights(name, wexp):
+    """Stokes weights of 2^-120 or 2^120 scale the normal matrix by 2^-240 or
+    2^240 (and chi2 with it): the SVD's bidiagonal then lies outside
+    [2^-200, 2^200] and is rescaled by a power of two before the QR phase,
+    whose Givens pairs square their operands unscaled.  Two-trial cap, as in
+    the parity test above: agreement with the oracle pixel by pixel.  (Much
+    further out, near 2^+-300, the reference's own QR shift overflows or
+    underflows and its parameters turn to NaN.)"""
+    wt = (2.0 ** wexp,) * 4
+    ok, floor, out, ref = _parity(name, 512, "heuristic", weights=wt, lambda_limits=(1e-5, 6e-3))
+    print(f"{name} weights 2^{wexp}: {ok.mean():.4f} of pixels within tolerance (oracle self {floor:.4f})")
+    assert out["stats"].n_svd_fail == 0
+    assert ok.mean() >= 0.98
+    assert ok.mean() >= floor - 0.01
+
+
 @pytest.mark.parametrize("init_mode", ["heuristic", "explicit"])
 @pytest.mark.parametrize("name", ["niris64", "hinode", "hmi4k"])
 def test_inversion_fp64_parity_default_policy(name, init_mode):
@@ -234,8 +252,8 @@ def test_region_counts_switch_changes_counters_only():
 
 def test_autotuner_picks_a_valid_shape_and_keeps_results():
     """me_ctx_autotune: the chosen block size and resident waves are one of
-    the candidates, stay in the context, and the inversion is the same under
-    every launch shape."""
+    the candidates (every SIMD of a CU holds a wave: four per CU), stay in the
+    context, and the inversion is the same under every launch shape."""
     import torch
     w = get("hmi4k")
     n = 20000
@@ -250,6 +268,7 @@ def test_autotuner_picks_a_valid_shape_and_keeps_results():
         print(tune)
         assert tune["block_size"] in (64, 128, 256) and tune["waves_per_cu"] >= 1 and tune["kernel_ms"] > 0
         assert tune["waves_per_cu"] % (tune["block_size"] // 64) == 0
+        assert tune["waves_per_cu"] >= 4
         d_res.zero_()
         inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
         np.testing.assert_array_equal(d_res.cpu().numpy(), before)
