@@ -55,6 +55,7 @@ struct me_ctx {
     DevCfg *d_cfg = nullptr;  // device copy of dc, read by the inversion kernel
     DevBuf scratch, prof, res, fit, init, pxs, par, out, jac, fitpar;
     DevBuf okey, oidx, otmp;  // learned mode's work-queue order: keys, pixel indices, sort storage
+    DevBuf grec;              // per-pixel start records (guess_kernel)
     // pinned staging blocks of me_invert_batch's host copies and the events
     // of their last DMA (kStageThreads x 2)
     void *pin[4][2] = {};
@@ -207,6 +208,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
     int block, grid;
     persistent_grid(c, kern, p1 - p0, block, grid);
     double *scratch = nullptr;
+    const me::GuessRec *grec = nullptr;
     if (S.ext_scratch) {
         const int64_t cap = me::kNoAllocLanes / block;
         if (grid > cap) {
@@ -219,6 +221,18 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
     } else {
         HIPCHK(S.scratch->ensure(sizeof(double) * me::SL_N * (int64_t)grid * block));
         scratch = (double *)S.scratch->p;
+        // Init_Guess and the validity test of every pixel, one thread each,
+        // ahead of the inversion (the allocation-free path keeps them in the
+        // inversion kernel)
+        const int64_t gn = p1 - p0;
+        if (gn > 0) {
+            HIPCHK(c->grec.ensure(sizeof(me::GuessRec) * (size_t)p1));
+            const int gb = 256;
+            hipLaunchKernelGGL(me::guess_kernel<TIn>, dim3((unsigned)((gn + gb - 1) / gb)), dim3(gb), 0, S.stream,
+                               (const DevCfg *)c->d_cfg, d_prof, p0, p1, d_init, learned, (me::GuessRec *)c->grec.p);
+            HIPCHK(hipGetLastError());
+            grec = (const me::GuessRec *)c->grec.p;
+        }
     }
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, S.stream, S.queue, 1, (unsigned long long)p0);
     HIPCHK(hipGetLastError());
@@ -232,6 +246,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
     A.res = d_res;
     A.fitpar = d_fitpar;
     A.init = d_init;
+    A.grec = grec;
     A.learned = learned;
     A.scratch = scratch;
     A.nslots = (int64_t)grid * block;
@@ -682,7 +697,7 @@ void me_ctx_destroy(me_ctx *c)
     for (void *p : owned)
         if (p) (void)hipFree(p);
     DevBuf *bufs[] = {&c->scratch, &c->prof, &c->res, &c->fit, &c->init, &c->pxs, &c->par, &c->out, &c->jac, &c->fitpar,
-                      &c->okey, &c->oidx, &c->otmp};
+                      &c->okey, &c->oidx, &c->otmp, &c->grec};
     for (DevBuf *b : bufs) b->release();
     for (auto &row : c->pin)
         for (void *p : row)

@@ -40,6 +40,7 @@ struct InvArgs {
     double *res;
     double *fitpar;  // [n][9] parameters of the best fit before HMI_REF (fit profiles are synthesised from them)
     const double *init;
+    const struct GuessRec *grec;  // per-pixel start records (guess_kernel), or null: Init_Guess in the kernel
     int learned;
     double *scratch;
     int64_t nslots;
@@ -165,6 +166,59 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
     l7hi = Imax * 1.4;
     step7 = Imax * 0.1;
     return G;
+}
+
+// Start record of one pixel: the validity test of INVERSION_MULTI
+// (Src/LM_FIT.c:781-790), Init_Guess and its S limits and step
+// (Src/ME_SOLVER.c:445-676), Noise_Init's norm0 (Src/ME_SOLVER.c:410-441)
+// and normp; g is the bounds-checked heuristic start, or the explicit one.
+// Written for every pixel by guess_kernel before the inversion launch, so a
+// lane that takes a pixel loads 128 contiguous bytes instead of re-reading
+// the profile and running Init_Guess while the rest of its wave waits.
+struct GuessRec {
+    double g[9];
+    double l7lo, l7hi, step7, norm0, normp;
+    double valid;   // 1: invert, 0: zero record
+    double pad;
+};
+
+// one thread per pixel of [p0, n)
+template <typename TIn>
+__global__ __launch_bounds__(256) void guess_kernel(const DevCfg *__restrict__ Cp, const void *__restrict__ prof,
+                                                   int64_t p0, int64_t n, const double *__restrict__ init,
+                                                   int learned, GuessRec *__restrict__ rec)
+{
+    CfgRef C = cfg_of(Cp);
+    const int64_t pix = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n) return;
+    const int nl = C.nl;
+    const int64_t base = pix * 4 * nl;
+    double mean = 0.0;
+    for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(prof, base + iw);
+    mean /= nl;
+    GuessRec R;
+    const bool valid = !(isnan(mean) || mean < C.icrit) && (C.nruns > 0 || learned);
+    R.valid = valid ? 1.0 : 0.0;
+    R.pad = 0.0;
+    if (valid) {
+        const Guess G = init_guess<TIn>(Cp, prof, pix);
+#pragma unroll
+        for (int k = 0; k < 9; k++) R.g[k] = (init && !learned) ? init[pix * 9 + k] : G.g[k];
+        R.l7lo = G.l7lo;
+        R.l7hi = G.l7hi;
+        R.step7 = G.step7;
+        R.norm0 = G.norm0;
+        R.normp = G.normp;
+        bounds_check(C, R.g, R.l7lo, R.l7hi);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) R.g[k] = 0.0;
+        R.l7lo = R.l7hi = R.step7 = R.norm0 = R.normp = 0.0;
+    }
+    double4 *dst = reinterpret_cast<double4 *>(rec + pix);
+    const double *src = reinterpret_cast<const double *>(&R);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dst[q] = make_double4(src[4 * q], src[4 * q + 1], src[4 * q + 2], src[4 * q + 3]);
 }
 
 __device__ __forceinline__ double penalty_of(CfgRef C, const double p[9])
@@ -616,11 +670,26 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             // the continuum so that both arrive in one round trip
             double ini[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) ini[k] = A.init ? A.init[pix * 9 + k] : 0.0;
-            double mean = 0.0;
-            for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(A.prof, base + iw);
-            mean /= nl;
-            const bool valid = !(isnan(mean) || mean < C.icrit) && (C.nruns > 0 || A.learned);
+            for (int k = 0; k < 9; k++) ini[k] = (A.init && (A.learned || !A.grec)) ? A.init[pix * 9 + k] : 0.0;
+            double rv[16];
+            bool valid;
+            if (A.grec) {
+                const double4 *rp = reinterpret_cast<const double4 *>(A.grec + pix);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double4 v = rp[q];
+                    rv[4 * q] = v.x;
+                    rv[4 * q + 1] = v.y;
+                    rv[4 * q + 2] = v.z;
+                    rv[4 * q + 3] = v.w;
+                }
+                valid = rv[14] != 0.0;
+            } else {
+                double mean = 0.0;
+                for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(A.prof, base + iw);
+                mean /= nl;
+                valid = !(isnan(mean) || mean < C.icrit) && (C.nruns > 0 || A.learned);
+            }
             if (!valid) {
 #pragma unroll
                 for (int k = 0; k < 10; k++) A.res[pix * 10 + k] = 0.0;
@@ -636,19 +705,30 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 }
                 continue;  // stays PH_ACQ
             }
-            const Guess G = init_guess<TIn>(Cp, A.prof, pix);
-            double gs[9], norm0 = G.norm0;
+            double gs[9], norm0;
+            if (A.grec) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) gs[k] = G.g[k];
-            l7lo = G.l7lo;
-            l7hi = G.l7hi;
-            step7 = G.step7;
-            normp = G.normp;
-            if (A.init && !A.learned) {
+                for (int k = 0; k < 9; k++) gs[k] = rv[k];
+                l7lo = rv[9];
+                l7hi = rv[10];
+                step7 = rv[11];
+                norm0 = rv[12];
+                normp = rv[13];
+            } else {
+                const Guess G = init_guess<TIn>(Cp, A.prof, pix);
+                norm0 = G.norm0;
 #pragma unroll
-                for (int k = 0; k < 9; k++) gs[k] = ini[k];
+                for (int k = 0; k < 9; k++) gs[k] = G.g[k];
+                l7lo = G.l7lo;
+                l7hi = G.l7hi;
+                step7 = G.step7;
+                normp = G.normp;
+                if (A.init && !A.learned) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) gs[k] = ini[k];
+                }
+                bounds_check(C, gs, l7lo, l7hi);
             }
-            bounds_check(C, gs, l7lo, l7hi);
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 SCR(SL_GUESS + k) = gs[k];
