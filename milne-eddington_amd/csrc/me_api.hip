@@ -168,10 +168,29 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
     st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
 }
 
-// Learned mode's work-queue order (start_chi_kernel): pixels sorted by the
-// chi2 at their starting point, worst first.  Returns the order (queue
-// position -> pixel) in the context's buffers, or null when n does not fit
-// 32-bit indices.
+// Start records of the pixels [p0, p1) (guess_kernel) in the context's
+// buffer; with key/idx non-null also the learned work-queue keys
+template <typename TIn>
+me_status enqueue_guess(me_ctx *c, hipStream_t s, const void *d_prof, int64_t p0, int64_t p1, const double *d_init,
+                        int learned, float *key, int32_t *idx)
+{
+    const int64_t gn = p1 - p0;
+    if (gn <= 0) return ME_OK;
+    HIPCHK(c->grec.ensure(sizeof(me::GuessRec) * (size_t)p1));
+    const int P = me::guess_pixels_per_block<TIn>(c->nl);
+    const size_t smem = (size_t)P * (4 * c->nl + 1) * sizeof(TIn);
+    hipLaunchKernelGGL(me::guess_kernel<TIn>, dim3((unsigned)((gn + P - 1) / P)), dim3(64), smem, s,
+                       (const DevCfg *)c->d_cfg, d_prof, p0, p1, P, d_init, learned, (me::GuessRec *)c->grec.p, key,
+                       idx);
+    HIPCHK(hipGetLastError());
+    return ME_OK;
+}
+
+// Learned mode's work-queue order: the start records and, for every pixel,
+// the chi2 at its learned starting point (guess_kernel), then the pixels
+// sorted by it, worst first.  Returns the order (queue position -> pixel) in
+// the context's buffers, or null when n does not fit 32-bit indices (then no
+// records were written either).
 template <typename TIn>
 me_status learned_order(me_ctx *c, hipStream_t s, const void *d_prof, const double *d_init, int64_t n,
                         const int32_t **order)
@@ -182,10 +201,8 @@ me_status learned_order(me_ctx *c, hipStream_t s, const void *d_prof, const doub
     HIPCHK(c->oidx.ensure(sizeof(int32_t) * 2 * n));
     float *key = (float *)c->okey.p;
     int32_t *idx = (int32_t *)c->oidx.p;
-    const int b = 256;
-    hipLaunchKernelGGL(me::start_chi_kernel<TIn>, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, s, (const DevCfg *)c->d_cfg, d_prof,
-                       d_init, n, key, idx);
-    HIPCHK(hipGetLastError());
+    me_status st = enqueue_guess<TIn>(c, s, d_prof, 0, n, d_init, 1, key, idx);
+    if (st != ME_OK) return st;
     size_t tmp = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, key, key + n, idx, idx + n, (int)n, 0, 32, s));
     HIPCHK(c->otmp.ensure(tmp > 0 ? tmp : 1));
@@ -202,7 +219,7 @@ me_status learned_order(me_ctx *c, hipStream_t s, const void *d_prof, const doub
 template <typename T, typename TIn>
 me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p0, int64_t p1, int64_t npx,
                          double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px,
-                         const int32_t *order = nullptr)
+                         const int32_t *order = nullptr, bool records_ready = false)
 {
     auto kern = me::invert_kernel<T, TIn>;
     int block, grid;
@@ -221,16 +238,14 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
     } else {
         HIPCHK(S.scratch->ensure(sizeof(double) * me::SL_N * (int64_t)grid * block));
         scratch = (double *)S.scratch->p;
-        // Init_Guess and the validity test of every pixel, one thread each,
-        // ahead of the inversion (the allocation-free path keeps them in the
-        // inversion kernel)
-        const int64_t gn = p1 - p0;
-        if (gn > 0) {
-            HIPCHK(c->grec.ensure(sizeof(me::GuessRec) * (size_t)p1));
-            const int gb = 256;
-            hipLaunchKernelGGL(me::guess_kernel<TIn>, dim3((unsigned)((gn + gb - 1) / gb)), dim3(gb), 0, S.stream,
-                               (const DevCfg *)c->d_cfg, d_prof, p0, p1, d_init, learned, (me::GuessRec *)c->grec.p);
-            HIPCHK(hipGetLastError());
+        // Init_Guess and the validity test of every pixel ahead of the
+        // inversion (the allocation-free path keeps them in the inversion
+        // kernel); learned_order has written them already when it ran
+        if (p1 > p0) {
+            if (!records_ready) {
+                me_status st = enqueue_guess<TIn>(c, S.stream, d_prof, p0, p1, d_init, learned, nullptr, nullptr);
+                if (st != ME_OK) return st;
+            }
             grec = (const me::GuessRec *)c->grec.p;
         }
     }
@@ -290,7 +305,7 @@ me_status launch_invert(me_ctx *c, const void *d_prof, int64_t n, double *d_res,
     me_status s = ME_OK;
     if (learned && d_init && !ext_scratch && (s = learned_order<TIn>(c, c->stream, d_prof, d_init, n, &order)) != ME_OK)
         return s;
-    s = enqueue_invert<T, TIn>(c, S, d_prof, 0, n, n, d_res, fitpar, d_init, learned, d_px, order);
+    s = enqueue_invert<T, TIn>(c, S, d_prof, 0, n, n, d_res, fitpar, d_init, learned, d_px, order, order != nullptr);
     if (s != ME_OK) return s;
     if (d_fit && (s = enqueue_fit<T>(c, c->stream, fitpar, 0, n, d_fit)) != ME_OK) return s;
     unsigned long long ct[me::CT_N];

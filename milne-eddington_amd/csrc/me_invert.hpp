@@ -64,31 +64,54 @@ struct Guess {
     double l7lo, l7hi, step7, norm0, normp;
 };
 
-template <typename TIn>
-__device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t pix)
+// Profile readers of Init_Guess: value at offset off from the pixel's first
+// sample (I[0]); off may be negative for the read before the I block at
+// small nl (H7), which reaches the previous pixel, or NaN before the first
+// pixel of the batch.
+template <typename TIn> struct GlobalRd {
+    const void *prof;
+    int64_t base;
+    __device__ __forceinline__ double operator()(int off) const
+    {
+        const int64_t i = base + off;
+        return i < 0 ? (double)NAN : ld<TIn>(prof, i);
+    }
+};
+// the pixel's 4 nl samples staged in LDS (row), earlier offsets from memory
+template <typename TIn> struct StagedRd {
+    const TIn *row;
+    const void *prof;
+    int64_t base;
+    __device__ __forceinline__ double operator()(int off) const
+    {
+        if (off >= 0) return (double)row[off];
+        const int64_t i = base + off;
+        return i < 0 ? (double)NAN : ld<TIn>(prof, i);
+    }
+};
+
+template <typename Rd>
+__device__ __forceinline__ Guess init_guess_r(CfgRef C, const Rd &rd)
 {
-    CfgRef C = cfg_of(Cp);
-    prof = sgpr_ptr(prof);
     Guess G;
     double *g = G.g;
     double &l7lo = G.l7lo, &l7hi = G.l7hi, &step7 = G.step7, &norm0 = G.norm0, &normp = G.normp;
     const int nl = C.nl;
-    const int64_t base = pix * 4 * nl;
     double sI = 0., Imin = 1e10, sQ = 0., sU = 0., sV = 0.;
     double Imax = 0, Lmax = 0., Vmax = 0., Vmin = 0.;
     int iVmax = 0, iVmin = 0, iL = 0;
     for (int iw = 0; iw < nl; iw++) {
-        double I = ld<TIn>(prof, base + iw);
+        double I = rd(iw);
         if (Imax < I) Imax = I;
         if (Imin > I) { iL = iw; Imin = I; }
         sI += I;
-        sQ += fabs(ld<TIn>(prof, base + nl + iw));
-        sU += fabs(ld<TIn>(prof, base + 2 * nl + iw));
-        sV += fabs(ld<TIn>(prof, base + 3 * nl + iw));
+        sQ += fabs(rd(nl + iw));
+        sU += fabs(rd(2 * nl + iw));
+        sV += fabs(rd(3 * nl + iw));
     }
     for (int iw = C.i0; iw <= C.i1; iw++) {
-        double q = ld<TIn>(prof, base + nl + iw), u = ld<TIn>(prof, base + 2 * nl + iw);
-        double vv = ld<TIn>(prof, base + 3 * nl + iw);
+        double q = rd(nl + iw), u = rd(2 * nl + iw);
+        double vv = rd(3 * nl + iw);
         double lp = sqrt(q * q + u * u);
         if (Lmax < lp) Lmax = lp;
         if (Vmin > vv) { iVmin = iw; Vmin = vv; }
@@ -121,10 +144,9 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
         if (Imin < 0.9 * Imax && iL >= C.ncut && iL < nl - C.ncut) {
             // at small nl the reference reads I[iL-2] before the pixel's I
             // block (the previous pixel's V); before the first pixel: NaN
-            const int64_t im2 = base + iL - 2;
-            double Im2 = im2 < 0 ? (double)NAN : ld<TIn>(prof, im2);
-            double Im1 = ld<TIn>(prof, base + iL - 1), I0 = ld<TIn>(prof, base + iL);
-            double Ip1 = ld<TIn>(prof, base + iL + 1), Ip2 = ld<TIn>(prof, base + iL + 2);
+            double Im2 = rd(iL - 2);
+            double Im1 = rd(iL - 1), I0 = rd(iL);
+            double Ip1 = rd(iL + 1), Ip2 = rd(iL + 2);
             if (Im1 < Im2 && I0 < Im1 && I0 < Ip1 && Ip1 < Ip2) {
                 dl = uniform_ro(C.lam)[iL] - lam0;
                 ok = true;
@@ -135,7 +157,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
             int a = (C.ic > 7) ? (C.ic - 7) : 0;
             int b = (C.ic + 7 < nl) ? (C.ic + 7) : (nl - 1);
             for (int iw = a; iw <= b; iw++) {
-                const double d = (Imax - ld<TIn>(prof, base + iw));
+                const double d = (Imax - rd(iw));
                 num += uniform_ro(C.lam)[iw] * d;
                 den += d;
             }
@@ -148,7 +170,7 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
     if (g[0] > 3000) g[0] = 3000;
     double wsum = 0., phs = 0.;
     for (int iw = C.i0; iw <= C.i1; iw++) {
-        double q = ld<TIn>(prof, base + nl + iw), u = ld<TIn>(prof, base + 2 * nl + iw);
+        double q = rd(nl + iw), u = rd(2 * nl + iw);
         double lp = sqrt(q * q + u * u);
         if (lp > thL) {
             double t = 0.5 * atan2(u, q);
@@ -168,6 +190,13 @@ __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const vo
     return G;
 }
 
+template <typename TIn>
+__device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t pix)
+{
+    CfgRef C = cfg_of(Cp);
+    return init_guess_r(C, GlobalRd<TIn>{sgpr_ptr(prof), pix * 4 * C.nl});
+}
+
 // Start record of one pixel: the validity test of INVERSION_MULTI
 // (Src/LM_FIT.c:781-790), Init_Guess and its S limits and step
 // (Src/ME_SOLVER.c:445-676), Noise_Init's norm0 (Src/ME_SOLVER.c:410-441)
@@ -182,26 +211,49 @@ struct GuessRec {
     double pad;
 };
 
-// one thread per pixel of [p0, n)
+// Start records of the pixels [p0, n).  A block of 64 threads stages the
+// samples of P consecutive pixels in LDS with coalesced loads (each pixel's
+// row padded to 4 nl + 1 values, an odd length, so that lanes reading their
+// own rows hit distinct banks); threads 0 .. P-1 then run Init_Guess on one
+// pixel each from LDS.  (One thread per pixel reading memory directly made a
+// wave's loads touch 64 cache lines each and overflow L1: 8.4 ms for 1M
+// pixels at nl = 60.)
+// With key non-null (learned mode), it also writes the work-queue key of
+// each pixel: the chi2 of the FP32 synthesis at the bounds-checked learned
+// start, in units of Noise_Init's (0.01 mean I)^2 (-1 for a pixel that is not
+// inverted), and idx = the pixel; sorting the keys, worst first, orders the
+// queue.  Only the order depends on them, never a pixel's record.
 template <typename TIn>
-__global__ __launch_bounds__(256) void guess_kernel(const DevCfg *__restrict__ Cp, const void *__restrict__ prof,
-                                                   int64_t p0, int64_t n, const double *__restrict__ init,
-                                                   int learned, GuessRec *__restrict__ rec)
+__global__ __launch_bounds__(64) void guess_kernel(const DevCfg *__restrict__ Cp, const void *__restrict__ prof,
+                                                  int64_t p0, int64_t n, int P, const double *__restrict__ init,
+                                                  int learned, GuessRec *__restrict__ rec, float *__restrict__ key,
+                                                  int32_t *__restrict__ idx)
 {
+    extern __shared__ double guess_smem[];
+    TIn *S = reinterpret_cast<TIn *>(guess_smem);
     CfgRef C = cfg_of(Cp);
-    const int64_t pix = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= n) return;
-    const int nl = C.nl;
-    const int64_t base = pix * 4 * nl;
+    const int nl = C.nl, n4 = 4 * nl, stride = n4 + 1;
+    const int64_t pb = p0 + (int64_t)blockIdx.x * P;
+    const int np = (int)((n - pb) < (int64_t)P ? (n - pb) : (int64_t)P);
+    const gptr<const TIn> src = global_ptr(reinterpret_cast<const TIn *>(prof) + pb * n4);
+    for (int k = threadIdx.x; k < np * n4; k += 64) {
+        const int j = k / n4, e = k - j * n4;
+        S[j * stride + e] = src[k];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= np) return;
+    const int64_t pix = pb + t;
+    const TIn *row = S + t * stride;
     double mean = 0.0;
-    for (int iw = 0; iw < nl; iw++) mean += ld<TIn>(prof, base + iw);
+    for (int iw = 0; iw < nl; iw++) mean += (double)row[iw];
     mean /= nl;
     GuessRec R;
     const bool valid = !(isnan(mean) || mean < C.icrit) && (C.nruns > 0 || learned);
     R.valid = valid ? 1.0 : 0.0;
     R.pad = 0.0;
     if (valid) {
-        const Guess G = init_guess<TIn>(Cp, prof, pix);
+        const Guess G = init_guess_r(C, StagedRd<TIn>{row, prof, pix * n4});
 #pragma unroll
         for (int k = 0; k < 9; k++) R.g[k] = (init && !learned) ? init[pix * 9 + k] : G.g[k];
         R.l7lo = G.l7lo;
@@ -216,9 +268,45 @@ __global__ __launch_bounds__(256) void guess_kernel(const DevCfg *__restrict__ C
         R.l7lo = R.l7hi = R.step7 = R.norm0 = R.normp = 0.0;
     }
     double4 *dst = reinterpret_cast<double4 *>(rec + pix);
-    const double *src = reinterpret_cast<const double *>(&R);
+    const double *sv = reinterpret_cast<const double *>(&R);
 #pragma unroll
-    for (int q = 0; q < 4; q++) dst[q] = make_double4(src[4 * q], src[4 * q + 1], src[4 * q + 2], src[4 * q + 3]);
+    for (int q = 0; q < 4; q++) dst[q] = make_double4(sv[4 * q], sv[4 * q + 1], sv[4 * q + 2], sv[4 * q + 3]);
+    if (key) {
+        idx[pix] = (int32_t)pix;
+        float kv = -1.f;
+        if (valid) {
+            double par[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) par[k] = init[pix * 9 + k];
+            // the learned run's S bound (gs[7] = Imax)
+            bounds_check(C, par, R.l7lo, C.ls7 > 0 ? C.ls7 * R.g[7] : R.l7hi);
+            PixK<float> K;
+            pix_consts<float>(C, par, K);
+            float chi = 0.f;
+            for (int iw = 0; iw < nl; iw++) {
+                float o[4];
+                me_point<float, false>(C, K, 0.f, iw, o, NoSink{}, (FCount *)nullptr);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float d = o[q] - (float)row[q * nl + iw];
+                    chi += d * d * (float)C.wsq[q];
+                }
+            }
+            const double nz = 1e-2 * mean;
+            kv = (float)(chi / (nz * nz * 4.0 * nl));
+            kv = isfinite(kv) ? kv : 3.0e38f;
+        }
+        key[pix] = kv;
+    }
+}
+
+// pixels per block of guess_kernel: 64, or fewer when 64 staged rows exceed
+// 32 KiB of LDS
+template <typename TIn> inline int guess_pixels_per_block(int nl)
+{
+    const int row = (4 * nl + 1) * (int)sizeof(TIn);
+    const int p = 32768 / row;
+    return p < 1 ? 1 : (p > 64 ? 64 : p);
 }
 
 __device__ __forceinline__ double penalty_of(CfgRef C, const double p[9])
@@ -359,15 +447,25 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
     return pos;
 }
 
+// Fixed wavelength parts of the FP32 path's shared passes: about 14
+// wavelengths each (2 to 8 parts; nl = 60: 4, nl = 112: 8).  Part h covers
+// [h nl / S, (h + 1) nl / S).
+__device__ __forceinline__ int split_parts(int nl)
+{
+    const int s = nl / 14;
+    return s < 2 ? 2 : (s > 8 ? 8 : s);
+}
+
 // The Jacobian pass of the FP32 path spread over the whole wave.  The
-// wavelengths of a pixel are cut into two fixed halves [0, nl/2) and
-// [nl/2, nl); its normal matrix, gradient and chi2 are the sum of the two
-// halves' partial sums (each in wavelength order), whichever lanes compute
-// them, so results do not depend on how pixels fall into waves.  The lanes
-// that need a normal matrix (`own`) are ranked r = 0 .. nJ-1; work item
-// i = h nJ + r (half h of pixel r) goes to lane i mod 64 in batch i div 64:
-// one batch of nl/2 wavelengths while nJ <= 32, instead of nl per lane.
-// Owners add the halves in order and write them like jac_pass.  Returns the
+// wavelengths of a pixel are cut into S fixed parts (split_parts); its normal
+// matrix, gradient and chi2 are the sums of the parts' partial sums (each in
+// wavelength order) added in part order, whichever lanes compute them, so
+// results do not depend on how pixels fall into waves.  The lanes that need a
+// normal matrix (`own`) are ranked r = 0 .. nJ-1; work item i = h nJ + r
+// (part h of pixel r) goes to lane i mod 64 in batch i div 64: while
+// nJ <= 64 / S one batch of nl / S wavelengths instead of nl per lane (a
+// pixel restarted alone at the end of a launch runs its passes S times
+// faster).  Owners add the parts and write them like jac_pass.  Returns the
 // owner's chi2 and this lane's own Faddeeva branch counts.  Called by the
 // whole wave.
 template <typename T, typename TIn>
@@ -379,7 +477,7 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
     prof = sgpr_ptr(prof);
     const gptr<double> mat = global_ptr(mat_), gout = global_ptr(gout_);
     const int nl = C.nl;
-    const int nh = nl >= 2 ? 2 : 1;                           // halves per pixel
+    const int nh = split_parts(nl);                           // parts per pixel
     const uint64_t mask = __ballot(own);
     const int nJ = __popcll(mask);
     const int lane = __lane_id();
@@ -402,8 +500,8 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
         for (int q = 0; q < 9; q++) p.v[q] = __shfl(par.v[q], src);
         const double s7 = __shfl(step7, src), iv = __shfl(ivn, src);
         const int64_t b = __shfl(base, src);
-        const int l0 = work ? (h == 0 ? 0 : nl / 2) : 0;
-        const int l1 = work ? (h == 0 && nh == 2 ? nl / 2 : nl) : 0;
+        const int l0 = work ? h * nl / nh : 0;
+        const int l1 = work ? (h + 1) * nl / nh : 0;
         PixK<T> K;
         pix_consts<T>(C, p.v, K);
         const T ivT[4] = {T(C.wsq[0] * iv), T(C.wsq[1] * iv), T(C.wsq[2] * iv), T(C.wsq[3] * iv)};
@@ -441,12 +539,12 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
                 sink(s, row);
             }, &fc);
         }
-        // the owner of rank rk takes half 0 (item rk, always in batch 0) and
-        // then half 1 (item nJ + rk) when they fall into this batch
-#pragma unroll
-        for (int hh = 0; hh < 2; hh++) {
+        // the owner of rank rk takes part hh (item hh nJ + rk) when it falls
+        // into this batch; parts arrive in increasing order
+        for (int hh = 0; hh < nh; hh++) {
             const int it = hh * nJ + rk;
-            const bool take = own && hh < nh && it / 64 == t;
+            const bool take = own && it / 64 == t;
+            if (__ballot(take) == 0) continue;
             const int from = it % 64;
 #pragma unroll
             for (int q = 0; q < 45; q++) {
@@ -473,6 +571,74 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
                 idx++;
             }
             gout[(int64_t)a * NS] = (double)G[a];
+        }
+    }
+    return PassRes{chs / (4. * nl), fc.lo, fc.hi};
+}
+
+// The trial synthesis of the FP32 path spread over the whole wave, like
+// jac_split: the lanes with a trial (`own`) are ranked, each pixel's
+// wavelengths cut into the same S fixed parts, and the owner adds the parts'
+// chi2 in part order.  Called by the whole wave.
+template <typename T, typename TIn>
+__device__ __noinline__ PassRes trial_split(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, bool own,
+                                            int64_t base, P9 pt, double ivn)
+{
+    CfgRef C = cfg_of(Cp);
+    prof = sgpr_ptr(prof);
+    const int nl = C.nl;
+    const int nh = split_parts(nl);
+    const uint64_t mask = __ballot(own);
+    const int nJ = __popcll(mask);
+    const int lane = __lane_id();
+    const int rk = __popcll(mask & lanemask_lt());
+    const int batches = (nh * nJ + 63) / 64;
+    double chs = 0.0;
+    FCount fc;
+    for (int t = 0; t < batches; t++) {
+        const int i = 64 * t + lane;
+        const int h = i / nJ, r = i % nJ;
+        const bool work = i < nh * nJ;
+        const int src = nth_set_bit(mask, r);
+        P9 p;
+#pragma unroll
+        for (int q = 0; q < 9; q++) p.v[q] = __shfl(pt.v[q], src);
+        const double iv = __shfl(ivn, src);
+        const int64_t b = __shfl(base, src);
+        const int l0 = work ? h * nl / nh : 0;
+        const int l1 = work ? (h + 1) * nl / nh : 0;
+        PixK<T> K;
+        pix_consts<T>(C, p.v, K);
+        double chi = 0.0;
+        T pn[4] = {T(0), T(0), T(0), T(0)}, ln = T(0);
+        if (l0 < l1) {
+            ln = wavelength_entry<T>(C, l0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + l0));
+        }
+        for (int iw = l0; iw < l1; iw++) {
+            T o[4], pc[4];
+            const T lc = ln;
+#pragma unroll
+            for (int q = 0; q < 4; q++) pc[q] = pn[q];
+            if (iw + 1 < l1) {
+                ln = wavelength_entry<T>(C, iw + 1);
+#pragma unroll
+                for (int q = 0; q < 4; q++) pn[q] = T(ld<TIn>(prof, b + q * nl + iw + 1));
+            }
+            me_point_at<T, false>(C, K, T(0), iw, lc, o, NoSink{}, &fc);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const T d = o[q] - pc[q];
+                chi += (double)d * (double)d * (C.wsq[q] * iv);
+            }
+        }
+        for (int hh = 0; hh < nh; hh++) {
+            const int it = hh * nJ + rk;
+            const bool take = own && it / 64 == t;
+            if (__ballot(take) == 0) continue;
+            const double v = __shfl(chi, it % 64);
+            if (take) chs = hh == 0 ? v : chs + v;
         }
     }
     return PassRes{chs / (4. * nl), fc.lo, fc.hi};
@@ -822,7 +988,9 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
         }
 
         // ------------------------------------------------------------ trial
-        if (phase == PH_TRIAL && !ended) {
+        const bool tl = phase == PH_TRIAL && !ended;
+        double pt[9];
+        if (tl) {
             // SVD of the input matrix (the normal matrix, or U of the previous
             // rejected trial: H1); U goes to the other slot and becomes the
             // next trial's input.  One call site for every lane of the wave.
@@ -835,7 +1003,6 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
 #pragma unroll
             for (int k = 0; k < 9; k++) sol[k] = sv.x[k];
             mbuf ^= 1;
-            double pt[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) pt[k] = par[k];
             if (C.npar == 9) {
@@ -854,16 +1021,34 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 }
             }
             bounds_check(C, pt, l7lo, l7hi);
-
+        }
+        // trial synthesis: shared by the wave on the FP32 path at nl >= 12
+        // (trial_split, every lane counts the Faddeeva calls it made), else per
+        // lane in the reference's order
+        PassRes tr{0.0, 0, 0};
+        if (split) {
+            if (__ballot(tl) != 0) {
+                P9 pv;
+#pragma unroll
+                for (int k = 0; k < 9; k++) pv.v[k] = pt[k];
+                tr = trial_split<T, TIn>(Cp, A.prof, tl, base, pv, ivn);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((tr.flo >> (16 * b)) & 0xFFFF);
+#pragma unroll
+                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((tr.fhi >> (16 * b)) & 0xFFFF);
+            }
+        } else if (tl) {
             P9 pv;
 #pragma unroll
             for (int k = 0; k < 9; k++) pv.v[k] = pt[k];
-            const PassRes tr = trial_pass<T, TIn>(Cp, A.prof, base, pv, ivn);
-            const double chi = tr.chi;
+            tr = trial_pass<T, TIn>(Cp, A.prof, base, pv, ivn);
 #pragma unroll
             for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((tr.flo >> (16 * b)) & 0xFFFF);
 #pragma unroll
             for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((tr.fhi >> (16 * b)) & 0xFFFF);
+        }
+        if (tl) {
+            const double chi = tr.chi;
             CT(CT_TRIAL) += 1;
             px_trials++;
             lmchisq = chi;
@@ -1013,54 +1198,6 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
     }
 #undef CT
 #undef SCR
-}
-
-// Work-queue priority of a pixel in learned mode: chi2 of the FP32 synthesis at
-// the (bounds-checked) starting point, in units of the pixel's noise
-// (Noise_Init's (0.01 mean I)^2).  A poor start predicts a long LM run and a
-// heuristic fallback; handing those pixels out first keeps them from being
-// the last ones running when the queue drains.  Pixels that are not
-// inverted get -1 (last).  Only the order changes, never a pixel's result.
-template <typename TIn>
-__global__ __launch_bounds__(256) void start_chi_kernel(const DevCfg *__restrict__ Cp, const void *prof, const double *init, int64_t n,
-                                                        float *key, int32_t *idx)
-{
-    CfgRef C = cfg_of(Cp);
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    idx[p] = (int32_t)p;
-    const int nl = C.nl;
-    const int64_t base = p * 4 * nl;
-    double imax = 0.0, isum = 0.0;
-    for (int iw = 0; iw < nl; iw++) {
-        const double I = ld<TIn>(prof, base + iw);
-        imax = imax < I ? I : imax;
-        isum += I;
-    }
-    const double imean = isum / nl;
-    if (!(imean >= C.icrit)) {
-        key[p] = -1.f;
-        return;
-    }
-    double par[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) par[k] = init[p * 9 + k];
-    bounds_check(C, par, 0.6 * imax, 1.4 * imax);
-    PixK<float> K;
-    pix_consts<float>(C, par, K);
-    float chi = 0.f;
-    for (int iw = 0; iw < nl; iw++) {
-        float o[4];
-        me_point<float, false>(C, K, 0.f, iw, o, NoSink{}, (FCount *)nullptr);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float d = o[q] - (float)ld<TIn>(prof, base + q * nl + iw);
-            chi += d * d * (float)C.wsq[q];
-        }
-    }
-    const double nz = 1e-2 * imean;
-    const float k = (float)(chi / (nz * nz * 4.0 * nl));
-    key[p] = isfinite(k) ? k : 3.0e38f;
 }
 
 // forward synthesis, one pixel per thread
