@@ -106,12 +106,11 @@ def test_inversion_fp64_parity_two_trial_cap(name):
 @pytest.mark.parametrize("name", ["niris64", "hmi4k"])
 def test_inversion_fp64_parity_extreme_weights(name, wexp):
     """Stokes weights of 2^-120 or 2^120 scale the normal matrix by 2^-240 or
-    2^240 (and chi2 with it): the SVD's bidiagonal then lies outside
-    [2^-200, 2^200] and is rescaled by a power of two before the QR phase,
-    whose Givens pairs square their operands unscaled.  Two-trial cap, as in
-    the parity test above: agreement with the oracle pixel by pixel.  (Much
-    further out, near 2^+-300, the reference's own QR shift overflows or
-    underflows and its parameters turn to NaN.)"""
+    2^240 (and chi2 with it): the SVD, its Givens pairs and the solve must
+    handle magnitudes far from 1.  Two-trial cap, as in the parity test above:
+    agreement with the oracle pixel by pixel.  (Much further out, near
+    2^+-300, the reference's own QR shift overflows or underflows and its
+    parameters turn to NaN.)"""
     wt = (2.0 ** wexp,) * 4
     ok, floor, out, ref = _parity(name, 512, "heuristic", weights=wt, lambda_limits=(1e-5, 6e-3))
     print(f"{name} weights 2^{wexp}: {ok.mean():.4f} of pixels within tolerance (oracle self {floor:.4f})")
