@@ -877,6 +877,15 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         QW(i) = w[i];
         QE(i) = e[i];
     }
+    // negligibility: |x| + anorm == anorm holds exactly when |x| <= tneg,
+    // half an ulp of anorm when its last mantissa bit is even (the tie rounds
+    // to anorm), the next double below it when odd (0 when anorm is 0 or in
+    // the smallest binades): one compare per test instead of an add and a
+    // compare
+    double tneg = __builtin_amdgcn_ldexp(1.0, __builtin_amdgcn_frexp_exp(anorm) - 54);
+    if ((__double_as_longlong(anorm) & 1ll) != 0 && tneg > 0.0)
+        tneg = __longlong_as_double(__double_as_longlong(tneg) - 1);
+    if (anorm == 0.0) tneg = 0.0;
     int fail = 0;
     int k = 8, its = 0;
     unsigned convm = 0;   // bit i: singular value i converged
@@ -898,8 +907,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         __builtin_amdgcn_sched_group_barrier(0x0002, 200, 0);
 #pragma unroll
         for (int ll = 0; ll < 9; ll++) {
-            nme |= (fabs(ev[ll]) + anorm == anorm) ? (1u << ll) : 0u;
-            nmw |= (fabs(wv[ll]) + anorm == anorm) ? (2u << ll) : 0u;
+            nme |= (fabs(ev[ll]) <= tneg) ? (1u << ll) : 0u;
+            nmw |= (fabs(wv[ll]) <= tneg) ? (2u << ll) : 0u;
         }
         // split search from l = k down (e[l] tested before w[l-1]); lanes
         // whose k converged without a cancellation take the next k
@@ -939,7 +948,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                     double f = s * ei;
                     ev[i] = c * ei;
                     QE(i) = ev[i];
-                    if (fabs(f) + anorm == anorm) {
+                    if (fabs(f) <= tneg) {
                         stop = true;
                     } else {
                         double gg = wv[i];
