@@ -79,7 +79,12 @@ typedef struct me_config {
     int      init_mode;        /* ME_INIT_HEURISTIC | ME_INIT_LEARNED       */
     uint64_t seed;             /* Philox4x32-10 key of the restart jumps    */
     int      block_size;       /* 0 = 256; else 64/128/256 (me_ctx_autotune) */
-    int      waves_per_cu;     /* 0 = all that fit; resident waves per CU    */
+    /* resident waves per CU.  The FP32 kernel has a build for one wave per
+     * SIMD (4 per CU) and one for two (8 per CU): above 4 selects the
+     * two-wave build.  0 = all that fit of the default build: the two-wave
+     * one for heuristic starts, the one-wave one for learned starts and the
+     * allocation-free call.  The FP64 path has the one-wave build only. */
+    int      waves_per_cu;
     /* upper damping bound of the learned run only (ME_INIT_LEARNED); 6e-3
      * by default, 0 = lambda_limits[1].  The damping never enters the solve
      * (trial k >= 2 of an iteration solves with the previous trial's U), it
@@ -159,8 +164,8 @@ me_status me_ctx_set_region_counts(me_ctx *ctx, int on);
 
 /* Launch-shape autotuner of the persistent inversion kernel: inverts the
  * first n_sample device profiles under every candidate block size (64, 128,
- * 256 lanes) and resident-wave count per CU that the occupancy allows and
- * that gives each SIMD of a CU a wave (best of two runs each, the context's
+ * 256 lanes) and resident-wave count per CU (one wave per SIMD, and on the
+ * FP32 path two) that the occupancy allows (best of two runs each, the context's
  * init mode and precision), keeps the fastest in the context's block_size /
  * waves_per_cu and reports it.  The sample should hold some 64 pixels per
  * resident lane (4M pixels on 256 CUs): a smaller one is timed by its slowest
@@ -198,7 +203,7 @@ me_status me_invert_device(me_ctx *ctx, const void *d_prof, int prof_is_f64, int
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * The kernel's per-lane scratch is a static device array of the library
- * (9 + 2*81 doubles for each of at most 65536 resident lanes, 89.6 MB per GPU,
+ * (190 doubles for each of at most 65536 resident lanes, 99.6 MB per GPU,
  * held until the process ends) instead of the context's buffer; the work and
  * event counters are the context's and are cleared by a fill kernel.  Same
  * results as me_invert_device.  Limits: no best-fit profiles; learned mode

@@ -168,6 +168,31 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
     st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
 }
 
+// The FP32 inversion kernel has two builds: for one wave per SIMD (a lane's
+// registers are the whole 512-entry file) and for two (256 each; the FP32 SVD
+// fits, the parameters wait in LDS).  A waves_per_cu above one wave per SIMD
+// selects the two-wave build.  By default (waves_per_cu = 0) heuristic starts
+// take the two-wave build and learned starts the one-wave build: the learned
+// map's fallback pixels, whose restarts run one after another, are at the
+// head of the queue, and at half the issue rate per lane they end later than
+// the rest.  The allocation-free path's static scratch holds the one-wave
+// launch's lanes.  The FP64 path has the one-wave build only.
+int invert_build(const me_ctx *c, int learned, bool ext_scratch)
+{
+    if (c->cfg.precision == ME_FP64) return 1;
+    if (c->cfg.waves_per_cu > 0) return c->cfg.waves_per_cu > kSimdsPerCU ? 2 : 1;
+    return (learned || ext_scratch) ? 1 : 2;
+}
+
+using InvertKernel = void (*)(const DevCfg *, me::InvArgs);
+template <typename T, typename TIn> InvertKernel invert_kernel_for(int wps)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (wps == 2) return me::invert_kernel<T, TIn, 2>;
+    }
+    return me::invert_kernel<T, TIn, 1>;
+}
+
 // Start records of the pixels [p0, p1) (guess_kernel) in the context's
 // buffer; with key/idx non-null also the learned work-queue keys
 template <typename TIn>
@@ -221,7 +246,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
                          double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px,
                          const int32_t *order = nullptr, bool records_ready = false)
 {
-    auto kern = me::invert_kernel<T, TIn>;
+    const auto kern = invert_kernel_for<T, TIn>(invert_build(c, learned, S.ext_scratch != nullptr));
     int block, grid;
     persistent_grid(c, kern, p1 - p0, block, grid);
     double *scratch = nullptr;
@@ -331,10 +356,11 @@ me_status invert_dispatch(me_ctx *c, const void *d_prof, int is_f64, int64_t n, 
 }
 
 // ---- the allocation-free entry point's scratch
-// The per-lane scratch of the inversion kernel (gradient + two 9x9 matrix
-// slots, me::SL_N doubles per lane, structure of arrays) for at most
+// The per-lane scratch of the inversion kernel (gradient, two 9x9 matrix
+// slots, the heuristic start and the learned run's result: me::SL_N doubles
+// per lane, structure of arrays) for at most
 // me::kNoAllocLanes resident lanes, as a static device array: each GPU the
-// library runs on holds its own copy (SL_N * kNoAllocLanes * 8 B = 89.6 MB)
+// library runs on holds its own copy (SL_N * kNoAllocLanes * 8 B = 99.6 MB)
 // until the process ends.  Host code never passes `scratch` itself to a
 // kernel (on the host it is only a stand-in for the symbol): its device
 // address is resolved per device by hipGetSymbolAddress and kept here.
@@ -838,24 +864,32 @@ me_status me_ctx_autotune(me_ctx *c, const void *d_prof, int prof_is_f64, int64_
 {
     if (!c || n <= 0 || !d_prof) return ME_EINVAL;
     if (c->cfg.init_mode == ME_INIT_LEARNED && !c->mlp.trained) return ME_ENOTREADY;
-    // resident waves per CU that each block size allows (registers, LDS)
-    auto kern = c->cfg.precision == ME_FP64
-                    ? (prof_is_f64 ? (const void *)me::invert_kernel<double, double> : (const void *)me::invert_kernel<double, float>)
-                    : (prof_is_f64 ? (const void *)me::invert_kernel<float, double> : (const void *)me::invert_kernel<float, float>);
-    // Only shapes that give every SIMD of a CU at least one wave: a shape that
-    // leaves SIMDs idle can only win on a sample whose time is set by its
-    // slowest pixels (a tail of restarted fallback pixels running alone),
-    // and then loses by the idle fraction on a full map.  The kernel's LDS
-    // columns are sized for 256 lanes, so smaller blocks fit once per CU and
-    // drop out unless the occupancy allows them enough waves.
+    // Candidates: one wave per SIMD (the one-wave build) and, on the FP32
+    // path, two (the two-wave build), for each block size whose resident
+    // blocks (registers, LDS) reach that many waves per CU.  Only shapes that
+    // give every SIMD of a CU at least one wave: a shape that leaves SIMDs
+    // idle can only win on a sample whose time is set by its slowest pixels
+    // (a tail of restarted fallback pixels running alone), and then loses by
+    // the idle fraction on a full map.  The kernel's LDS columns are sized for
+    // 256 lanes, so small blocks drop out when LDS caps their residency.
+    auto kern_of = [&](int wps) -> const void * {
+        if (c->cfg.precision == ME_FP64)
+            return prof_is_f64 ? (const void *)invert_kernel_for<double, double>(1)
+                               : (const void *)invert_kernel_for<double, float>(1);
+        return prof_is_f64 ? (const void *)invert_kernel_for<float, double>(wps)
+                           : (const void *)invert_kernel_for<float, float>(wps);
+    };
     struct Cand { int block, waves; };
     std::vector<Cand> cands;
-    for (int block : {64, 128, 256}) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-        const int wmax = per_cu * (block / 64);
-        for (int waves = wmax; waves >= block / 64 && waves >= wmax / 2; waves -= block / 64)
-            if (waves >= kSimdsPerCU) cands.push_back({block, waves});
+    const int max_wps = c->cfg.precision == ME_FP64 ? 1 : 2;
+    for (int wps = 1; wps <= max_wps; wps++) {
+        const int waves = wps * kSimdsPerCU;
+        for (int block : {64, 128, 256}) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern_of(wps), block, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            if (waves % (block / 64) == 0 && per_cu * (block / 64) >= waves) cands.push_back({block, waves});
+        }
     }
     if (cands.empty()) cands.push_back({256, 0});
     double *d_res = nullptr;

@@ -3,12 +3,13 @@
 // Layout and mapping: one pixel per lane.  A lane walks its pixel's nl
 // wavelengths serially, so nl = 6, 60 and 112 all keep 64 lanes busy; the
 // Jacobian of a wavelength lives in registers only and is folded into the
-// 9x9 normal matrix as soon as it is formed.  The 9x9 SVD runs in FP64 on the
-// lane, with every array index a compile-time constant (or selected from a
+// 9x9 normal matrix as soon as it is formed.  The 9x9 SVD runs on the lane,
+// with every array index a compile-time constant (or selected from a
 // compile-time range), so the matrices stay in VGPRs/AGPRs instead of scratch.
 //
-// Compute precision T is float or double for synthesis, Faddeeva, Jacobian
-// and normal matrix; the SVD solve and chi2 accept/reject are always double.
+// Compute precision T is float or double for synthesis, Faddeeva, Jacobian,
+// normal matrix and SVD solve; chi2 and the accept/reject test are always
+// double.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -704,14 +705,54 @@ __device__ __forceinline__ double rsq_nr(double q)  // q normal, finite, > 0
 // left 18 of 5e9 SVDs on hmi4k maps without convergence in 50 sweeps, against
 // none with the scaling; a wave-uniform fallback to the scaled form cost more
 // than it saved.
-__device__ __forceinline__ void hyp_cs(double f, double h, double &z, double &cf, double &ch)
+//
+// The SVD runs in S = double (FP64 path) or S = float (FP32 path).  FP32
+// uses the hardware reciprocal and reciprocal square root (about 1 ulp) as
+// they are.  neg_bound(anorm) is the largest |x| with |x| + anorm == anorm:
+// half an ulp of anorm when its last mantissa bit is even (the tie rounds to
+// anorm), the next value below it when odd, 0 when anorm is 0 or so small
+// that the half ulp is below the smallest (or, in FP32, normal) value.
+struct SvdF64 {
+    static __device__ __forceinline__ double abs_(double x) { return fabs(x); }
+    static __device__ __forceinline__ double max_(double a, double b) { return fmax(a, b); }
+    static __device__ __forceinline__ double rcp(double x) { return rcp_nr(x); }
+    static __device__ __forceinline__ double rsq(double x) { return rsq_nr(x); }
+    static __device__ __forceinline__ int fexp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+    static __device__ __forceinline__ double ldx(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+    static __device__ __forceinline__ double neg_bound(double anorm)
+    {
+        double t = __builtin_amdgcn_ldexp(1.0, __builtin_amdgcn_frexp_exp(anorm) - 54);
+        if ((__double_as_longlong(anorm) & 1ll) != 0 && t > 0.0) t = __longlong_as_double(__double_as_longlong(t) - 1);
+        return anorm == 0.0 ? 0.0 : t;
+    }
+};
+struct SvdF32 {
+    static __device__ __forceinline__ float abs_(float x) { return fabsf(x); }
+    static __device__ __forceinline__ float max_(float a, float b) { return fmaxf(a, b); }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+    static __device__ __forceinline__ int fexp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+    static __device__ __forceinline__ float ldx(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+    static __device__ __forceinline__ float neg_bound(float anorm)
+    {
+        float t = __builtin_amdgcn_ldexpf(1.0f, __builtin_amdgcn_frexp_expf(anorm) - 25);
+        if ((__float_as_int(anorm) & 1) != 0 && t > 0.0f) t = __int_as_float(__float_as_int(t) - 1);
+        return anorm == 0.0f ? 0.0f : t;
+    }
+};
+template <typename S> struct SvdOps;
+template <> struct SvdOps<double> : SvdF64 {};
+template <> struct SvdOps<float> : SvdF32 {};
+
+template <typename S> __device__ __forceinline__ void hyp_cs(S f, S h, S &z, S &cf, S &ch)
 {
-    const double mx = fmax(fabs(f), fabs(h));
-    const int ex = __builtin_amdgcn_frexp_exp(mx);   // mx = m 2^ex, m in [0.5, 1)
-    const double fs = __builtin_amdgcn_ldexp(f, -ex), hs = __builtin_amdgcn_ldexp(h, -ex);
-    const double q = fs * fs + hs * hs;
-    const double r = rsq_nr(fmax(q, 0.25));
-    z = __builtin_amdgcn_ldexp(q * r, ex);
+    using O = SvdOps<S>;
+    const S mx = O::max_(O::abs_(f), O::abs_(h));
+    const int ex = O::fexp(mx);   // mx = m 2^ex, m in [0.5, 1)
+    const S fs = O::ldx(f, -ex), hs = O::ldx(h, -ex);
+    const S q = fs * fs + hs * hs;
+    const S r = O::rsq(O::max_(q, S(0.25)));
+    z = O::ldx(q * r, ex);
     cf = fs * r;
     ch = hs * r;
 }
@@ -738,15 +779,24 @@ template <int B, int E, typename F> __device__ __forceinline__ void sfor_down(F 
 }
 
 // w and e of the QR phase, one column per lane of the block (blocks of at
-// most 256 lanes)
+// most 256 lanes), in the SVD's precision
 __shared__ double g_svd_w[9][256];
 __shared__ double g_svd_e[9][256];
+__shared__ float g_svd_wf[9][256];
+__shared__ float g_svd_ef[9][256];
+template <typename S> __device__ __forceinline__ S *svd_w_lds();
+template <> __device__ __forceinline__ double *svd_w_lds<double>() { return &g_svd_w[0][0]; }
+template <> __device__ __forceinline__ float *svd_w_lds<float>() { return &g_svd_wf[0][0]; }
+template <typename S> __device__ __forceinline__ S *svd_e_lds();
+template <> __device__ __forceinline__ double *svd_e_lds<double>() { return &g_svd_e[0][0]; }
+template <> __device__ __forceinline__ float *svd_e_lds<float>() { return &g_svd_ef[0][0]; }
 
 // returns 0, 1 when a singular value needed 50 QR sweeps, 2 for a non-finite matrix
-__device__ __forceinline__ int svd9(double a[81], double w[9])
+template <typename S> __device__ __forceinline__ int svd9(S a[81], S w[9])
 {
-    double e[9];
-    double g = 0.0, scale = 0.0, anorm = 0.0;
+    using O = SvdOps<S>;
+    S e[9];
+    S g = S(0), scale = S(0), anorm = S(0);
     bool finite = true;
 #pragma unroll
     for (int i = 0; i < 81; i++) finite = finite && isfinite(a[i]);
@@ -756,37 +806,37 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     // a merge of two paths (over a thousand register copies per call).
     if (!finite) {
 #pragma unroll
-        for (int i = 0; i < 81; i++) a[i] = 0.0;
+        for (int i = 0; i < 81; i++) a[i] = S(0);
     }
     // 1. bidiagonalisation
     sfor_up<0, 9>([&](auto I) __attribute__((always_inline)) {
         constexpr int i = decltype(I)::value;
         constexpr int l = i + 1;
         e[i] = scale * g;
-        double s = 0.0;
-        g = 0.0;
-        scale = 0.0;
+        S s = S(0);
+        g = S(0);
+        scale = S(0);
 #pragma unroll
-        for (int k = i; k < 9; k++) scale += fabs(a[k * 9 + i]);
-        if (scale != 0.0) {
-            const double rs = rcp_nr(scale);
+        for (int k = i; k < 9; k++) scale += O::abs_(a[k * 9 + i]);
+        if (scale != S(0)) {
+            const S rs = O::rcp(scale);
 #pragma unroll
             for (int k = i; k < 9; k++) {
                 a[k * 9 + i] *= rs;
                 s += a[k * 9 + i] * a[k * 9 + i];
             }
-            double f = a[i * 9 + i];
-            const double rts = s * rsq_nr(s);   // s in [1/9, 1]: entries scaled by their 1-norm
-            g = f >= 0 ? -rts : rts;
-            double h = f * g - s;
+            S f = a[i * 9 + i];
+            const S rts = s * O::rsq(s);   // s in [1/9, 1]: entries scaled by their 1-norm
+            g = f >= S(0) ? -rts : rts;
+            S h = f * g - s;
             a[i * 9 + i] = f - g;
-            const double rh = rcp_nr(h);
+            const S rh = O::rcp(h);
             sfor_up<l, 9>([&](auto Jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(Jc)::value;
-                double d = 0.0;
+                S d = S(0);
 #pragma unroll
                 for (int k = i; k < 9; k++) d += a[k * 9 + i] * a[k * 9 + j];
-                double fc = d * rh;
+                S fc = d * rh;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += fc * a[k * 9 + i];
                 SVD_BARRIER();
@@ -795,31 +845,31 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             for (int k = i; k < 9; k++) a[k * 9 + i] *= scale;
         }
         w[i] = scale * g;
-        s = 0.0;
-        g = 0.0;
-        scale = 0.0;
+        s = S(0);
+        g = S(0);
+        scale = S(0);
         if constexpr (i != 8) {
 #pragma unroll
-            for (int k = l; k < 9; k++) scale += fabs(a[i * 9 + k]);
-            if (scale != 0.0) {
-                const double rs = rcp_nr(scale);
+            for (int k = l; k < 9; k++) scale += O::abs_(a[i * 9 + k]);
+            if (scale != S(0)) {
+                const S rs = O::rcp(scale);
 #pragma unroll
                 for (int k = l; k < 9; k++) {
                     a[i * 9 + k] *= rs;
                     s += a[i * 9 + k] * a[i * 9 + k];
                 }
-                double f = a[i * 9 + l];
-                const double rts = s * rsq_nr(s);
-                g = f >= 0 ? -rts : rts;
-                double h = f * g - s;
+                S f = a[i * 9 + l];
+                const S rts = s * O::rsq(s);
+                g = f >= S(0) ? -rts : rts;
+                S h = f * g - s;
                 a[i * 9 + l] = f - g;
-                double rv[9];
-                const double rh = rcp_nr(h);
+                S rv[9];
+                const S rh = O::rcp(h);
 #pragma unroll
                 for (int k = l; k < 9; k++) rv[k] = a[i * 9 + k] * rh;
                 sfor_up<l, 9>([&](auto Jr) __attribute__((always_inline)) {
                     constexpr int j = decltype(Jr)::value;
-                    double d = 0.0;
+                    S d = S(0);
 #pragma unroll
                     for (int k = l; k < 9; k++) d += a[j * 9 + k] * a[i * 9 + k];
 #pragma unroll
@@ -830,25 +880,25 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 for (int k = l; k < 9; k++) a[i * 9 + k] *= scale;
             }
         }
-        double t = fabs(w[i]) + fabs(e[i]);
+        S t = O::abs_(w[i]) + O::abs_(e[i]);
         anorm = anorm < t ? t : anorm;
     });
     // 3. left-hand transformations into a (U)
     sfor_down<0, 9>([&](auto I) __attribute__((always_inline)) {
         constexpr int i = decltype(I)::value;
         constexpr int l = i + 1;
-        double gg = w[i];
+        S gg = w[i];
 #pragma unroll
-        for (int j = l; j < 9; j++) a[i * 9 + j] = 0.0;
-        if (gg != 0.0) {
-            gg = rcp_nr(gg);
-            const double rii = rcp_nr(a[i * 9 + i]);
+        for (int j = l; j < 9; j++) a[i * 9 + j] = S(0);
+        if (gg != S(0)) {
+            gg = O::rcp(gg);
+            const S rii = O::rcp(a[i * 9 + i]);
             sfor_up<l, 9>([&](auto Jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(Jc)::value;
-                double d = 0.0;
+                S d = S(0);
 #pragma unroll
                 for (int k = l; k < 9; k++) d += a[k * 9 + i] * a[k * 9 + j];
-                double f = (d * rii) * gg;
+                S f = (d * rii) * gg;
 #pragma unroll
                 for (int k = i; k < 9; k++) a[k * 9 + j] += f * a[k * 9 + i];
                 SVD_BARRIER();
@@ -857,9 +907,9 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             for (int j = i; j < 9; j++) a[j * 9 + i] *= gg;
         } else {
 #pragma unroll
-            for (int j = i; j < 9; j++) a[j * 9 + i] = 0.0;
+            for (int j = i; j < 9; j++) a[j * 9 + i] = S(0);
         }
-        a[i * 9 + i] += 1.0;
+        a[i * 9 + i] += S(1);
     });
     // 4. implicit-shift QR on the bidiagonal form.  Every lane runs its own
     //    sequence of NR steps: its own current singular value k, split index L
@@ -869,7 +919,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     //    runs max(sum of steps) rather than sum over k of max(steps at k).
     //    w and e live in the lane's LDS columns during this phase, so entries
     //    at k, k - 1 and L are addressed instead of picked by nine-way selects.
-    double *W = &g_svd_w[0][threadIdx.x], *E = &g_svd_e[0][threadIdx.x];
+    S *W = svd_w_lds<S>() + threadIdx.x;
+    S *E = svd_e_lds<S>() + threadIdx.x;
 #define QW(i) W[(i) * 256]
 #define QE(i) E[(i) * 256]
 #pragma unroll
@@ -878,14 +929,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         QE(i) = e[i];
     }
     // negligibility: |x| + anorm == anorm holds exactly when |x| <= tneg,
-    // half an ulp of anorm when its last mantissa bit is even (the tie rounds
-    // to anorm), the next double below it when odd (0 when anorm is 0 or in
-    // the smallest binades): one compare per test instead of an add and a
-    // compare
-    double tneg = __builtin_amdgcn_ldexp(1.0, __builtin_amdgcn_frexp_exp(anorm) - 54);
-    if ((__double_as_longlong(anorm) & 1ll) != 0 && tneg > 0.0)
-        tneg = __longlong_as_double(__double_as_longlong(tneg) - 1);
-    if (anorm == 0.0) tneg = 0.0;
+    // one compare per test instead of an add and a compare
+    const S tneg = O::neg_bound(anorm);
     int fail = 0;
     int k = 8, its = 0;
     unsigned convm = 0;   // bit i: singular value i converged
@@ -897,7 +942,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         // sweep instead of one per pair.  ev / wv then mirror e and w for the
         // rest of the sweep at every index the cancellation and the rotation
         // steps read (those below the lane's k, which only they modify)
-        double ev[9], wv[9];
+        S ev[9], wv[9];
 #pragma unroll
         for (int ll = 0; ll < 9; ll++) {
             ev[ll] = QE(ll);
@@ -907,8 +952,8 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         __builtin_amdgcn_sched_group_barrier(0x0002, 200, 0);
 #pragma unroll
         for (int ll = 0; ll < 9; ll++) {
-            nme |= (fabs(ev[ll]) <= tneg) ? (1u << ll) : 0u;
-            nmw |= (fabs(wv[ll]) <= tneg) ? (2u << ll) : 0u;
+            nme |= (O::abs_(ev[ll]) <= tneg) ? (1u << ll) : 0u;
+            nmw |= (O::abs_(wv[ll]) <= tneg) ? (2u << ll) : 0u;
         }
         // split search from l = k down (e[l] tested before w[l-1]); lanes
         // whose k converged without a cancellation take the next k
@@ -930,29 +975,29 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
         if (__ballot(k >= 0) == 0) break;
         if (k >= 0 && cancel) {
             // rotate column L-1 against columns L..k of U
-            double col[9];
+            S col[9];
 #pragma unroll
-            for (int r = 0; r < 9; r++) col[r] = 0.0;
+            for (int r = 0; r < 9; r++) col[r] = S(0);
 #pragma unroll
             for (int cc = 0; cc < 8; cc++)
                 if (cc == L - 1) {
 #pragma unroll
                     for (int r = 0; r < 9; r++) col[r] = a[r * 9 + cc];
                 }
-            double c = 0.0, s = 1.0;
+            S c = S(0), s = S(1);
             bool stop = false;
 #pragma unroll
             for (int i = 1; i < 9; i++) {
                 if (!stop && i >= L && i <= k) {
-                    const double ei = ev[i];
-                    double f = s * ei;
+                    const S ei = ev[i];
+                    S f = s * ei;
                     ev[i] = c * ei;
                     QE(i) = ev[i];
-                    if (fabs(f) <= tneg) {
+                    if (O::abs_(f) <= tneg) {
                         stop = true;
                     } else {
-                        double gg = wv[i];
-                        double h, cf, cg;
+                        S gg = wv[i];
+                        S h, cf, cg;
                         hyp_cs(f, gg, h, cf, cg);
                         QW(i) = h;
                         wv[i] = h;
@@ -960,7 +1005,7 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         s = -cf;
 #pragma unroll
                         for (int r = 0; r < 9; r++) {
-                            double yv = col[r], zv = a[r * 9 + i];
+                            S yv = col[r], zv = a[r * 9 + i];
                             col[r] = yv * c + zv * s;
                             a[r * 9 + i] = zv * c - yv * s;
                         }
@@ -975,9 +1020,9 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                 }
         }
         if (k >= 0) {
-            double z = QW(k);
-            const double ek = QE(k), wkm1 = k > 0 ? QW(k - 1) : 0.0, ekm1 = k > 0 ? QE(k - 1) : 0.0;
-            double x = QW(L);
+            S z = QW(k);
+            const S ek = QE(k), wkm1 = k > 0 ? QW(k - 1) : S(0), ekm1 = k > 0 ? QE(k - 1) : S(0);
+            S x = QW(L);
             if (L == k) {
                 // converged right after a cancellation
                 convm |= 1u << k;
@@ -986,16 +1031,16 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
             } else {
                 if (its == 49) fail = 1;
                 its++;
-                double yv = wkm1, gg = ekm1, h = ek;
+                S yv = wkm1, gg = ekm1, h = ek;
                 // Wilkinson shift from the trailing 2x2 block; its divisions
-                // and hypot(f, 1) use the Newton-refined reciprocal and the
-                // branch-free hypot (the shift only steers convergence)
-                double f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) * rcp_nr(2.0 * h * yv);
-                double cf_, c1_;
-                hyp_cs(f, 1.0, gg, cf_, c1_);
-                double sg = f >= 0.0 ? gg : -gg;
-                f = ((x - z) * (x + z) + h * (yv * rcp_nr(f + sg) - h)) * rcp_nr(x);
-                double c = 1.0, s = 1.0;
+                // and hypot(f, 1) use the reciprocal and the branch-free hypot
+                // (the shift only steers convergence)
+                S f = ((yv - z) * (yv + z) + (gg - h) * (gg + h)) * O::rcp(S(2) * h * yv);
+                S cf_, c1_;
+                hyp_cs(f, S(1), gg, cf_, c1_);
+                S sg = f >= S(0) ? gg : -gg;
+                f = ((x - z) * (x + z) + h * (yv * O::rcp(f + sg) - h)) * O::rcp(x);
+                S c = S(1), s = S(1);
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     if (j < k && j >= L) {
@@ -1010,10 +1055,10 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         gg = gg * c - x * s;
                         h = yv * s;
                         yv *= c;
-                        double c2, s2;
+                        S c2, s2;
                         hyp_cs(f, h, z, c2, s2);
                         QW(j) = z;
-                        if (z != 0.0) {
+                        if (z != S(0)) {
                             c = c2;
                             s = s2;
                         }
@@ -1021,13 +1066,13 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
                         x = c * yv - s * gg;
 #pragma unroll
                         for (int r = 0; r < 9; r++) {
-                            double y2 = a[r * 9 + j], z2 = a[r * 9 + i];
+                            S y2 = a[r * 9 + j], z2 = a[r * 9 + i];
                             a[r * 9 + j] = y2 * c + z2 * s;
                             a[r * 9 + i] = z2 * c - y2 * s;
                         }
                     }
                 }
-                QE(L) = 0.0;
+                QE(L) = S(0);
                 QE(k) = f;
                 QW(k) = x;
                 // 50 sweeps without convergence: flagged, go on with k - 1
@@ -1042,16 +1087,16 @@ __device__ __forceinline__ int svd9(double a[81], double w[9])
     // reference's convergence step, deferred: no later step reads them)
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-        const double wi = QW(i);
-        w[i] = ((convm >> i) & 1u) != 0u && wi < 0.0 ? -wi : wi;
+        const S wi = QW(i);
+        w[i] = ((convm >> i) & 1u) != 0u && wi < S(0) ? -wi : wi;
     }
 #undef QW
 #undef QE
     if (!finite) {
 #pragma unroll
-        for (int i = 0; i < 81; i++) a[i] = 0.0;
+        for (int i = 0; i < 81; i++) a[i] = S(0);
 #pragma unroll
-        for (int i = 0; i < 9; i++) w[i] = 0.0;
+        for (int i = 0; i < 9; i++) w[i] = S(0);
         fail = 2;
     }
     return fail;

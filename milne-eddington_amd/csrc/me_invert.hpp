@@ -21,15 +21,17 @@ enum {
     CT_RUNS, CT_ITERS, CT_VALID, CT_CONV, CT_SVDFAIL, CT_FALLBACK, CT_N
 };
 
-// per-lane slots in LDS (one column per lane of the block)
-enum { SL_BEST = 0, SL_GUESS = 9, SL_LEARN = 18, SL_CHI_BEST = 27, SL_CHI_LEARN = 28, SL_LDS_N = 29 };
+// per-lane slots in LDS (one column per lane of the block): the best run's
+// parameters and chi2, the current parameters and the trial parameters
+enum { SL_BEST = 0, SL_CHI_BEST = 9, SL_PAR = 10, SL_PT = 19, SL_LDS_N = 28 };
 // per-lane slots in the scratch buffer (structure of arrays): the gradient,
 // then two 81-entry matrices: the trial's input A and its left factor U
-// alternate between them (mbuf selects the input)
-enum { SL_G = 0, SL_MAT = 9, SL_N = 9 + 162 };
+// alternate between them (mbuf selects the input); then the start parameters
+// of the heuristic runs (Init_Guess) and the learned run's result and chi2,
+// which only the ends of runs read
+enum { SL_G = 0, SL_MAT = 9, SL_GUESS = 171, SL_LEARN = 180, SL_CHI_LEARN = 189, SL_N = 190 };
 // lanes of the allocation-free entry point's static scratch: 256 CUs x 256
-// resident lanes (the kernel holds one 256-lane block, i.e. one wave per SIMD,
-// per CU)
+// resident lanes (one 256-lane block per CU: the one-wave build's launch)
 constexpr int64_t kNoAllocLanes = 65536;
 
 struct InvArgs {
@@ -190,7 +192,7 @@ __device__ __forceinline__ Guess init_guess_r(CfgRef C, const Rd &rd)
     return G;
 }
 
-template <typename TIn>
+template <typename TIn, int WPS>
 __device__ __noinline__ Guess init_guess(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t pix)
 {
     CfgRef C = cfg_of(Cp);
@@ -328,6 +330,20 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
+// event counter k of the wave += v summed over its active lanes (an LDS
+// atomic per lane: the active lanes may be any subset of the wave), and += the
+// number of active lanes (one LDS write by the first of them; the row belongs
+// to the wave)
+__device__ __forceinline__ void ct_add(unsigned long long *wct, int k, uint32_t v)
+{
+    atomicAdd(&wct[k], (unsigned long long)v);
+}
+__device__ __forceinline__ void ct_inc(unsigned long long *wct, int k)
+{
+    const uint64_t m = __ballot(1);
+    if (__lane_id() == __ffsll((unsigned long long)m) - 1) wct[k] += (unsigned long long)__popcll(m);
+}
+
 // one atomic per wave hands consecutive pixel indices to the lanes that ask
 __device__ __forceinline__ int64_t wave_acquire(unsigned long long *q, bool need)
 {
@@ -367,7 +383,14 @@ struct PassRes {
     uint64_t flo, fhi;
 };
 
-template <typename T, typename TIn>
+// a pass's Faddeeva call counts (eight 16-bit fields) into counters F0..F7
+__device__ __forceinline__ void ct_fadd(unsigned long long *wct, const PassRes &r)
+{
+#pragma unroll
+    for (int b = 0; b < 8; b++) ct_add(wct, CT_F0 + b, (uint32_t)(((b < 4 ? r.flo : r.fhi) >> (16 * (b & 3))) & 0xFFFFu));
+}
+
+template <typename T, typename TIn, int WPS>
 __device__ __noinline__ PassRes jac_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
                                          P9 par, double step7, double ivn, double *__restrict__ mat_,
                                          double *__restrict__ gout_, int64_t NS)
@@ -468,7 +491,7 @@ __device__ __forceinline__ int split_parts(int nl)
 // faster).  Owners add the parts and write them like jac_pass.  Returns the
 // owner's chi2 and this lane's own Faddeeva branch counts.  Called by the
 // whole wave.
-template <typename T, typename TIn>
+template <typename T, typename TIn, int WPS>
 __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, bool own,
                                           int64_t base, P9 par, double step7, double ivn, double *__restrict__ mat_,
                                           double *__restrict__ gout_, int64_t NS)
@@ -580,7 +603,7 @@ __device__ __noinline__ PassRes jac_split(const DevCfg *__restrict__ Cp, const v
 // jac_split: the lanes with a trial (`own`) are ranked, each pixel's
 // wavelengths cut into the same S fixed parts, and the owner adds the parts'
 // chi2 in part order.  Called by the whole wave.
-template <typename T, typename TIn>
+template <typename T, typename TIn, int WPS>
 __device__ __noinline__ PassRes trial_split(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, bool own,
                                             int64_t base, P9 pt, double ivn)
 {
@@ -645,7 +668,7 @@ __device__ __noinline__ PassRes trial_split(const DevCfg *__restrict__ Cp, const
 }
 
 // Trial synthesis at pt: chi2 only (Loss_Function of Src/LM_FIT.c:89-120)
-template <typename T, typename TIn>
+template <typename T, typename TIn, int WPS>
 __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const void *__restrict__ prof, int64_t base,
                                            P9 pt, double ivn)
 {
@@ -686,18 +709,21 @@ __device__ __noinline__ PassRes trial_pass(const DevCfg *__restrict__ Cp, const 
 // A^T U W^-1 W^+ U^T g; the singular-value threshold grows x5 until
 // |x| <= 0.5, else x is clipped.  Not inlined: the SVD gets the register file
 // to itself instead of sharing it with the rest of the LM state machine.
+// S is the SVD's precision (the path's T); the matrices stay FP64 in memory.
 struct Sol9 {
     double x[9];
     int status, nbk;
 };
 
+template <typename S>
 __device__ __forceinline__ Sol9 svd_solve_body(const gptr<const double> in, const gptr<double> uo,
                                                const gptr<const double> gv, int64_t NS, double thr0, int npar)
 {
+    using O = SvdOps<S>;
     Sol9 r;
-    double M[81], w[9];
+    S M[81], w[9];
 #pragma unroll
-    for (int k = 0; k < 81; k++) M[k] = in[(int64_t)k * NS];
+    for (int k = 0; k < 81; k++) M[k] = S(in[(int64_t)k * NS]);
     r.status = svd9(M, w);
     // With fixed parameters the system is the packed npar x npar one, padded
     // with zero rows and columns: every Golub-Reinsch step on the padded
@@ -706,47 +732,48 @@ __device__ __forceinline__ Sol9 svd_solve_body(const gptr<const double> in, cons
     // entries keeps the next trial's input (this U, H1) padded with zeros.
 #pragma unroll
     for (int i = 0; i < 9; i++)
-        if (i >= npar) M[i * 10] = 0.0;
-    double ug[9];
+        if (i >= npar) M[i * 10] = S(0);
+    S ug[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) {
-        double sacc = 0.0;
+        S sacc = S(0);
 #pragma unroll
-        for (int i = 0; i < 9; i++) sacc += M[i * 9 + j] * gv[(int64_t)i * NS];
+        for (int i = 0; i < 9; i++) sacc += M[i * 9 + j] * S(gv[(int64_t)i * NS]);
         ug[j] = sacc;
     }
 #pragma unroll
-    for (int k = 0; k < 81; k++) uo[(int64_t)k * NS] = M[k];
-    double wmax = w[0];
+    for (int k = 0; k < 81; k++) uo[(int64_t)k * NS] = double(M[k]);
+    S wmax = w[0];
 #pragma unroll
     for (int i = 1; i < 9; i++) wmax = wmax > w[i] ? wmax : w[i];
-    double rw[9];
+    const S wmin = S(1e-50);   // 0 in FP32: every positive w is inverted there
+    S rw[9];
 #pragma unroll
-    for (int j = 0; j < 9; j++) rw[j] = w[j] > 1e-50 ? rcp_nr(w[j]) : 0.0;
+    for (int j = 0; j < 9; j++) rw[j] = w[j] > wmin ? O::rcp(w[j]) : S(0);
     double thr = thr0;
     r.nbk = 0;
     for (int pass = 0; pass < 64; pass++) {
-        const double wt = wmax * thr;
-        double y[9];
+        const S wt = wmax * S(thr);
+        S y[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) {
-            if (w[j] < wt) w[j] = 0.;
-            y[j] = w[j] > 1e-50 ? (ug[j] * rw[j]) * rw[j] : 0.0;
+            if (w[j] < wt) w[j] = S(0);
+            y[j] = w[j] > wmin ? (ug[j] * rw[j]) * rw[j] : S(0);
         }
-        double uy[9];
+        S uy[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            double sacc = 0.0;
+            S sacc = S(0);
 #pragma unroll
             for (int j = 0; j < 9; j++) sacc += M[k * 9 + j] * y[j];
             uy[k] = sacc;
         }
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-            double sacc = 0.0;
+            S sacc = S(0);
 #pragma unroll
-            for (int k = 0; k < 9; k++) sacc += in[(int64_t)(k * 9 + i) * NS] * uy[k];
-            r.x[i] = sacc;
+            for (int k = 0; k < 9; k++) sacc += S(in[(int64_t)(k * 9 + i) * NS]) * uy[k];
+            r.x[i] = double(sacc);
         }
         r.nbk++;
         bool ok = true;
@@ -766,10 +793,11 @@ __device__ __forceinline__ Sol9 svd_solve_body(const gptr<const double> in, cons
 // The call boundary sits on a thin wrapper: with the body inlined into it the
 // allocator keeps the SVD in 256 VGPRs + 160 AGPRs and 144 B of spill per
 // lane, against 256 + 256 and 300 B when the body is the call target itself.
+template <typename S, int WPS>
 __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__restrict__ uo,
                                        const double *__restrict__ gv, int64_t NS, double thr0, int npar)
 {
-    return svd_solve_body(global_ptr(in), global_ptr(uo), global_ptr(gv), NS, thr0, npar);
+    return svd_solve_body<S>(global_ptr(in), global_ptr(uo), global_ptr(gv), NS, thr0, npar);
 }
 
 // Fixed parameters (Src/LM_FIT.c:316-347): keep the rows and columns of the
@@ -777,6 +805,7 @@ __device__ __noinline__ Sol9 svd_solve(const double *__restrict__ in, double *__
 // block of the lane's matrix slot (zeros elsewhere), and the gradient into its
 // first npar entries.  Each target entry reads a source at the same or a later
 // position, so the compaction runs in place in row-major order.
+template <int WPS>
 __device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__restrict__ mat_,
                                        double *__restrict__ g_, int64_t NS)
 {
@@ -792,35 +821,45 @@ __device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__
     for (int a = 0; a < 9; a++) g[(int64_t)a * NS] = a < np ? g[(int64_t)C.pmap[a] * NS] : 0.0;
 }
 
-template <typename T, typename TIn>
-__global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
+// WPS: resident waves per SIMD the kernel is built for.  1: each lane has
+// the whole register file (512 entries, VGPRs and AGPRs); 2: half of it,
+// which the FP32 SVD fits in (the FP64 path is built for one wave only).  The
+// __noinline__ phases carry WPS as well, only so that each build calls its
+// own copies, compiled under that build's register budget.
+template <typename T, typename TIn, int WPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS)))
+void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
 {
     CfgRef C = cfg_of(Cp);
     const int nl = C.nl;
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *scr = A.scratch + slot;
     const int64_t NS = A.nslots;
-    // the small per-lane state (best, guess and learned parameters, their
-    // chi2) lives in LDS, one column per lane; the gradient and the matrices
-    // stay in the scratch buffer
+    // the parameters (current, trial, best of the runs) live in LDS, one
+    // column per lane, so that no register holds them across the phase
+    // calls; the rest of the per-lane state stays in the scratch buffer
     __shared__ double lsl[SL_LDS_N][256];
     double *mysl = &lsl[0][threadIdx.x];
-#define SCR(k) mysl[(k) * 256]
-    // event counters live in LDS, one column per lane
-    __shared__ uint32_t lct[CT_N][256];
-    uint32_t *myct = &lct[0][threadIdx.x];
-#define CT(k) myct[(k) * 256]
-#pragma unroll
-    for (int k = 0; k < CT_N; k++) CT(k) = 0;
+#define LS(k) mysl[(k) * 256]
+#define PAR(k) LS(SL_PAR + (k))
+#define PT(k) LS(SL_PT + (k))
+#define GSL(k) scr[(int64_t)(k) * NS]
+    // event counters: one row per wave in LDS, added to by the wave's sums
+    // over its active lanes (blocks of at most 256 lanes)
+    __shared__ unsigned long long lct[4][CT_N];
+    unsigned long long *wct = lct[threadIdx.x >> 6];
+    if (__lane_id() < CT_N) wct[__lane_id()] = 0;
 
     int phase = PH_ACQ;
     int64_t pix = -1;
     int irun = 0, it = 0, lphase = 0, mbuf = 0;
-    double par[9];
     double chisq = 0, damp = 0, ratio = 1, lmchisq = 0, normp = 1, ivn = 1, step7 = 0, l7lo = 0, l7hi = 0;
     int32_t px_iters = 0, px_runs = 0, px_trials = 0, px_status = 0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) par[k] = 0;
+    for (int k = 0; k < 9; k++) {
+        PAR(k) = 0.0;
+        PT(k) = 0.0;
+    }
 
     for (;;) {
         // ---------------------------------------------------- acquire pixels
@@ -881,7 +920,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 norm0 = rv[12];
                 normp = rv[13];
             } else {
-                const Guess G = init_guess<TIn>(Cp, A.prof, pix);
+                const Guess G = init_guess<TIn, WPS>(Cp, A.prof, pix);
                 norm0 = G.norm0;
 #pragma unroll
                 for (int k = 0; k < 9; k++) gs[k] = G.g[k];
@@ -896,21 +935,20 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 bounds_check(C, gs, l7lo, l7hi);
             }
 #pragma unroll
-            for (int k = 0; k < 9; k++) {
-                SCR(SL_GUESS + k) = gs[k];
-                par[k] = gs[k];
-            }
+            for (int k = 0; k < 9; k++) GSL(SL_GUESS + k) = gs[k];
             if (A.learned) {
                 // the learned run may bound S wider than Init_Guess (gs[7] is
                 // Imax); the heuristic runs of a fallback restore its bound
                 if (C.ls7 > 0) l7hi = C.ls7 * gs[7];
 #pragma unroll
-                for (int k = 0; k < 9; k++) par[k] = ini[k];
-                bounds_check(C, par, l7lo, l7hi);
+                for (int k = 0; k < 9; k++) gs[k] = ini[k];
+                bounds_check(C, gs, l7lo, l7hi);
                 lphase = 1;
             } else {
                 lphase = 0;
             }
+#pragma unroll
+            for (int k = 0; k < 9; k++) PAR(k) = gs[k];
             ivn = 1. / norm0;
             irun = 0;
             lmchisq = INFINITY;
@@ -930,18 +968,17 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
         // summation order.
         const bool jl = phase == PH_JAC_START || phase == PH_JAC;
         const bool split = sizeof(T) == 4 && nl >= 12;
+        // Faddeeva call counts: always on the FP64 path, on request on FP32
+        const bool fcount = sizeof(T) == 8 || C.fregions != 0;
         PassRes jr{0.0, 0, 0};
         if (split) {
             if (__ballot(jl) != 0) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = par[k];
-                jr = jac_split<T, TIn>(Cp, A.prof, jl, base, pv, step7, ivn, scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
+                for (int k = 0; k < 9; k++) pv.v[k] = PAR(k);
+                jr = jac_split<T, TIn, WPS>(Cp, A.prof, jl, base, pv, step7, ivn, scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
                                        scr + (int64_t)SL_G * NS, NS);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
+                if (fcount) ct_fadd(wct, jr);
             }
         }
         if (jl) {
@@ -949,26 +986,26 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             if (!split) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = par[k];
-                jr = jac_pass<T, TIn>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((jr.flo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((jr.fhi >> (16 * b)) & 0xFFFF);
+                for (int k = 0; k < 9; k++) pv.v[k] = PAR(k);
+                jr = jac_pass<T, TIn, WPS>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
+                if (fcount) ct_fadd(wct, jr);
             }
             const double chi = jr.chi;
-            CT(CT_JAC) += 1;
-            CT(CT_HESS) += 1;
+            ct_inc(wct, CT_JAC);
+            ct_inc(wct, CT_HESS);
             if (phase == PH_JAC_START) {
                 chisq = chi;
                 ratio = 1.0;
                 it = 0;
                 if (C.regul_any) {
-                    const double pen = penalty_of(C, par);
+                    double pp[9];
+#pragma unroll
+                    for (int k = 0; k < 9; k++) pp[k] = PAR(k);
+                    const double pen = penalty_of(C, pp);
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
             }
-            if (C.npar < 9) pack_free(Cp, mat, scr + (int64_t)SL_G * NS, NS);
+            if (C.npar < 9) pack_free<WPS>(Cp, mat, scr + (int64_t)SL_G * NS, NS);
             // regularisation terms on the diagonal and the gradient of the
             // free parameters (Src/LM_FIT.c:373-393)
             if (C.regul_any) {
@@ -976,7 +1013,7 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 for (int k = 0; k < 9; k++) {
                     const int ip = C.pidx[k];
                     if (C.regul[k] && ip >= 0) {
-                        const double r = par[k] - C.rval[k];
+                        const double r = PAR(k) - C.rval[k];
                         scr[(int64_t)(SL_G + ip) * NS] -= (r * C.rw[k]) * ratio;
                         mat[(int64_t)(ip * 10) * NS] += C.rw[k] * ratio;
                     }
@@ -989,22 +1026,22 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
 
         // ------------------------------------------------------------ trial
         const bool tl = phase == PH_TRIAL && !ended;
-        double pt[9];
         if (tl) {
             // SVD of the input matrix (the normal matrix, or U of the previous
             // rejected trial: H1); U goes to the other slot and becomes the
             // next trial's input.  One call site for every lane of the wave.
-            const Sol9 sv = svd_solve(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
+            const Sol9 sv = svd_solve<T, WPS>(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
                                       scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr, C.npar);
-            CT(CT_SVD) += 1;
-            CT(CT_BK) += sv.nbk;
-            if (sv.status) { px_status |= 2; CT(CT_SVDFAIL) += 1; }
+            ct_inc(wct, CT_SVD);
+            ct_add(wct, CT_BK, sv.nbk);
+            if (sv.status) { px_status |= 2; ct_inc(wct, CT_SVDFAIL); }
             double sol[9];
 #pragma unroll
             for (int k = 0; k < 9; k++) sol[k] = sv.x[k];
             mbuf ^= 1;
+            double pt[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) pt[k] = par[k];
+            for (int k = 0; k < 9; k++) pt[k] = PAR(k);
             if (C.npar == 9) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) pt[k] += sol[k] * (k == 7 ? step7 : C.step[k]);
@@ -1021,6 +1058,8 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 }
             }
             bounds_check(C, pt, l7lo, l7hi);
+#pragma unroll
+            for (int k = 0; k < 9; k++) PT(k) = pt[k];
         }
         // trial synthesis: shared by the wave on the FP32 path at nl >= 12
         // (trial_split, every lane counts the Faddeeva calls it made), else per
@@ -1030,36 +1069,33 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             if (__ballot(tl) != 0) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = pt[k];
-                tr = trial_split<T, TIn>(Cp, A.prof, tl, base, pv, ivn);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((tr.flo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-                for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((tr.fhi >> (16 * b)) & 0xFFFF);
+                for (int k = 0; k < 9; k++) pv.v[k] = PT(k);
+                tr = trial_split<T, TIn, WPS>(Cp, A.prof, tl, base, pv, ivn);
+                if (fcount) ct_fadd(wct, tr);
             }
         } else if (tl) {
             P9 pv;
 #pragma unroll
-            for (int k = 0; k < 9; k++) pv.v[k] = pt[k];
-            tr = trial_pass<T, TIn>(Cp, A.prof, base, pv, ivn);
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + b) += (uint32_t)((tr.flo >> (16 * b)) & 0xFFFF);
-#pragma unroll
-            for (int b = 0; b < 4; b++) CT(CT_F0 + 4 + b) += (uint32_t)((tr.fhi >> (16 * b)) & 0xFFFF);
+            for (int k = 0; k < 9; k++) pv.v[k] = PT(k);
+            tr = trial_pass<T, TIn, WPS>(Cp, A.prof, base, pv, ivn);
+            if (fcount) ct_fadd(wct, tr);
         }
         if (tl) {
             const double chi = tr.chi;
-            CT(CT_TRIAL) += 1;
+            ct_inc(wct, CT_TRIAL);
             px_trials++;
             lmchisq = chi;
             if (chi < chisq) {
                 if (C.regul_any) {
-                    const double pen = penalty_of(C, pt);
+                    double pp[9];
+#pragma unroll
+                    for (int k = 0; k < 9; k++) pp[k] = PT(k);
+                    const double pen = penalty_of(C, pp);
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
                 const bool conv = ((chisq - chi) / chisq) < C.convg;
 #pragma unroll
-                for (int k = 0; k < 9; k++) par[k] = pt[k];
+                for (int k = 0; k < 9; k++) PAR(k) = PT(k);
                 chisq = chi;
                 if (conv) {
                     ended = true;
@@ -1078,35 +1114,35 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             const double crun = chisq / normp;
             px_runs++;
             px_iters += it;
-            CT(CT_RUNS) += 1;
-            CT(CT_ITERS) += it;
+            ct_inc(wct, CT_RUNS);
+            ct_add(wct, CT_ITERS, it);
             bool finish = false, restart = false;
             if (lphase == 1) {
                 // learned start: accept at chisq_criteria, else heuristic runs
                 // (none when num_run = 0: the learned record is kept)
                 if (crun <= C.crit || C.nruns == 0) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) SCR(SL_BEST + k) = par[k];
-                    SCR(SL_CHI_BEST) = crun;
+                    for (int k = 0; k < 9; k++) LS(SL_BEST + k) = PAR(k);
+                    LS(SL_CHI_BEST) = crun;
                     lphase = 0;
                     finish = true;
                 } else {
 #pragma unroll
                     for (int k = 0; k < 9; k++) {
-                        SCR(SL_LEARN + k) = par[k];
-                        par[k] = SCR(SL_GUESS + k);
+                        GSL(SL_LEARN + k) = PAR(k);
+                        PAR(k) = GSL(SL_GUESS + k);
                     }
-                    SCR(SL_CHI_LEARN) = crun;
-                    l7hi = SCR(SL_GUESS + 7) * 1.4;   // Init_Guess's bound (Imax * 1.4)
+                    GSL(SL_CHI_LEARN) = crun;
+                    l7hi = PAR(7) * 1.4;   // Init_Guess's bound (Imax * 1.4)
                     lphase = 2;
                     px_status |= 4;
-                    CT(CT_FALLBACK) += 1;
+                    ct_inc(wct, CT_FALLBACK);
                     irun = 0;
                     phase = PH_JAC_START;
                 }
             } else {
                 bool save = false, brk = false;
-                double cbest = SCR(SL_CHI_BEST);
+                double cbest = LS(SL_CHI_BEST);
                 if (crun < C.crit) {
                     save = true;
                     brk = (irun >= 4 || irun == C.nruns * 4);
@@ -1117,8 +1153,8 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
                 }
                 if (save) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) SCR(SL_BEST + k) = par[k];
-                    SCR(SL_CHI_BEST) = crun;
+                    for (int k = 0; k < 9; k++) LS(SL_BEST + k) = PAR(k);
+                    LS(SL_CHI_BEST) = crun;
                     cbest = crun;
                 }
                 if (irun >= C.nruns * 2 && cbest < C.crit * 2) brk = true;
@@ -1129,38 +1165,41 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
             if (restart) {
                 // Random_Jump (Src/LM_FIT.c:157-201), Philox draws for irun >= 4
                 const int r = irun;
+                double np[9];
                 if (r < 4) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) par[k] = SCR(SL_BEST + k);
-                    par[2] = SCR(SL_GUESS + 2) + kPi / 4. * r;
+                    for (int k = 0; k < 9; k++) np[k] = LS(SL_BEST + k);
+                    np[2] = GSL(SL_GUESS + 2) + kPi / 4. * r;
                 } else {
                     double nz[8];
                     gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 0, nz);
                     gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 1, nz + 4);
-                    par[0] = SCR(SL_GUESS + 0) * (1 + 0.6 * nz[0]);
-                    par[1] = SCR(SL_GUESS + 1) + kPi / 8. * nz[1];
-                    par[2] = SCR(SL_GUESS + 2) + kPi / 4. * (r + 0.3 * nz[2]);
-                    par[3] = SCR(SL_GUESS + 3) + 5. * nz[3];
-                    par[4] = SCR(SL_BEST + 4) * (1 + 0.2 * nz[4]);
-                    par[5] = SCR(SL_GUESS + 5) + 0.1 * nz[5];
-                    par[6] = SCR(SL_GUESS + 6) + 5 * nz[6];
-                    par[7] = SCR(SL_BEST + 7);
-                    par[8] = SCR(SL_GUESS + 8) + 0.1 * nz[7];
+                    np[0] = GSL(SL_GUESS + 0) * (1 + 0.6 * nz[0]);
+                    np[1] = GSL(SL_GUESS + 1) + kPi / 8. * nz[1];
+                    np[2] = GSL(SL_GUESS + 2) + kPi / 4. * (r + 0.3 * nz[2]);
+                    np[3] = GSL(SL_GUESS + 3) + 5. * nz[3];
+                    np[4] = LS(SL_BEST + 4) * (1 + 0.2 * nz[4]);
+                    np[5] = GSL(SL_GUESS + 5) + 0.1 * nz[5];
+                    np[6] = GSL(SL_GUESS + 6) + 5 * nz[6];
+                    np[7] = LS(SL_BEST + 7);
+                    np[8] = GSL(SL_GUESS + 8) + 0.1 * nz[7];
                 }
-                bounds_check(C, par, l7lo, l7hi);
+                bounds_check(C, np, l7lo, l7hi);
+#pragma unroll
+                for (int k = 0; k < 9; k++) PAR(k) = np[k];
                 phase = PH_JAC_START;
             }
             if (finish) {
                 double best[9];
 #pragma unroll
-                for (int k = 0; k < 9; k++) best[k] = SCR(SL_BEST + k);
-                double cbest = SCR(SL_CHI_BEST);
+                for (int k = 0; k < 9; k++) best[k] = LS(SL_BEST + k);
+                double cbest = LS(SL_CHI_BEST);
                 if (lphase == 2) {
-                    const double cl = SCR(SL_CHI_LEARN);
+                    const double cl = GSL(SL_CHI_LEARN);
                     if (cl < cbest) {
                         cbest = cl;
 #pragma unroll
-                        for (int k = 0; k < 9; k++) best[k] = SCR(SL_LEARN + k);
+                        for (int k = 0; k < 9; k++) best[k] = GSL(SL_LEARN + k);
                     }
                 }
                 if (best[1] > kPi) best[1] -= kPi;
@@ -1175,8 +1214,8 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
 #pragma unroll
                 for (int k = 0; k < 9; k++) A.res[pix * 10 + k] = best[k];
                 A.res[pix * 10 + 9] = cbest;
-                CT(CT_VALID) += 1;
-                if (cbest <= C.crit) CT(CT_CONV) += 1;
+                ct_inc(wct, CT_VALID);
+                if (cbest <= C.crit) ct_inc(wct, CT_CONV);
                 if (A.px_iters) {
                     A.px_iters[pix] = px_iters;
                     A.px_runs[pix] = px_runs;
@@ -1189,15 +1228,11 @@ __global__ __launch_bounds__(256) void invert_kernel(const DevCfg *__restrict__ 
     }
 
     // ---------------------------------------------------------- counters
-#pragma unroll
-    for (int k = 0; k < CT_N; k++) {
-        unsigned long long v = CT(k);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (__lane_id() == 0 && v) atomicAdd(&A.counters[k], v);
-    }
-#undef CT
-#undef SCR
+    if (__lane_id() < CT_N && wct[__lane_id()] != 0) atomicAdd(&A.counters[__lane_id()], wct[__lane_id()]);
+#undef LS
+#undef PAR
+#undef PT
+#undef GSL
 }
 
 // forward synthesis, one pixel per thread

@@ -147,3 +147,21 @@ def test_learned_two_trial_bound_keeps_the_records(learned_case):
     assert cb.mean() >= ca.mean() - 0.002
     assert same >= 0.99
     assert np.allclose(qa, qb, rtol=0.02)
+
+
+def test_learned_one_and_two_wave_builds_agree(learned_case):
+    """The FP32 kernel's two builds (one wave per SIMD, the learned default,
+    and two) give the same learned records bit for bit."""
+    import torch
+    w, params, prof, _, _ = learned_case
+    d_prof = torch.from_numpy(prof).to("cuda:0")
+    d_res = device_tensor((N, 10), torch.float64)
+    recs = []
+    for waves in (4, 8):
+        with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED, block_size=256,
+                           waves_per_cu=waves) as inv:
+            inv.mlp_import(params, w.ranges())
+            inv.invert_device(d_prof.data_ptr(), N, d_res.data_ptr())
+            recs.append(d_res.cpu().numpy())
+    assert np.mean(recs[0][:, 9] > 0) > 0.9
+    np.testing.assert_array_equal(recs[1], recs[0])

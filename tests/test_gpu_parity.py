@@ -251,8 +251,9 @@ def test_region_counts_switch_changes_counters_only():
 
 def test_autotuner_picks_a_valid_shape_and_keeps_results():
     """me_ctx_autotune: the chosen block size and resident waves are one of
-    the candidates (every SIMD of a CU holds a wave: four per CU), stay in the
-    context, and the inversion is the same under every launch shape."""
+    the candidates (every SIMD of a CU holds a wave: four or eight per CU),
+    stay in the context, and the inversion is the same under every launch
+    shape and in both FP32 builds (one and two waves per SIMD)."""
     import torch
     w = get("hmi4k")
     n = 20000
@@ -267,12 +268,12 @@ def test_autotuner_picks_a_valid_shape_and_keeps_results():
         print(tune)
         assert tune["block_size"] in (64, 128, 256) and tune["waves_per_cu"] >= 1 and tune["kernel_ms"] > 0
         assert tune["waves_per_cu"] % (tune["block_size"] // 64) == 0
-        assert tune["waves_per_cu"] >= 4
+        assert tune["waves_per_cu"] in (4, 8)
         d_res.zero_()
         inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
         np.testing.assert_array_equal(d_res.cpu().numpy(), before)
-    for block in (64, 128):
-        with gpu_inversion("hmi4k", precision=ME_FP32, block_size=block) as inv:
+    for block, waves in ((64, 0), (128, 0), (256, 4), (128, 4), (256, 8)):
+        with gpu_inversion("hmi4k", precision=ME_FP32, block_size=block, waves_per_cu=waves) as inv:
             d_res.zero_()
             inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
             np.testing.assert_array_equal(d_res.cpu().numpy(), before)
