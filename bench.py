@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--svd-fp32", type=int, default=-1, choices=[-1, 0, 1],
+                    help="FP32 path: SVD solve in FP32 (1) or FP64 (0); -1: FP32 for heuristic, FP64 for learned starts")
     ap.add_argument("--tune-pixels", type=int, default=1 << 22,
                     help="autotune the launch shape on this many pixels first, about 64 per resident lane "
                          "(0: keep --block/--waves-per-cu)")
@@ -237,7 +239,8 @@ def main():
     w = get(args.workload)
     n = w.n if args.pixels <= 0 else min(args.pixels, w.n)
     fp64 = args.precision == "fp64"
-    over = dict(precision=ME_FP64 if fp64 else ME_FP32, block_size=args.block, waves_per_cu=args.waves_per_cu)
+    over = dict(precision=ME_FP64 if fp64 else ME_FP32, block_size=args.block, waves_per_cu=args.waves_per_cu,
+                svd_fp32=args.svd_fp32)
     over["num_run"] = args.num_run if args.num_run else w.bench_num_run
     cfg = w.config(**over)
     over_l = dict(over, init_mode=ME_INIT_LEARNED, learned_lambda_limit=args.learned_lambda_limit,
@@ -294,6 +297,11 @@ def main():
         dphi = np.minimum(d[:, 2], np.pi - d[:, 2])
         return {"B_G": round(float(np.median(d[:, 0])), 3), "theta_deg": round(float(np.median(d[:, 1]) * DEG), 4),
                 "phi_deg": round(float(np.median(dphi) * DEG), 4), "vlos_m_s": round(float(np.median(d[:, 3]) * 1e3), 3)}
+
+    def svd_precision(learned):
+        if fp64:
+            return "fp64"
+        return "fp32" if args.svd_fp32 == 1 or (args.svd_fp32 < 0 and not learned) else "fp64"
 
     def summary(ms, sts, tune=None):
         last = sts[-1]
@@ -489,7 +497,8 @@ def main():
                    "pixels": n, "init": "learned MLP + LM, fallback plain" if headline == "learned" else "heuristic",
                    "num_run": cfg.num_run, "chisq_criteria": cfg.chisq_criteria, "precision": args.precision,
                    "learned_lambda_limit": args.learned_lambda_limit if headline == "learned" else None,
-                   "learned_s_limit": args.learned_s_limit if headline == "learned" else None},
+                   "learned_s_limit": args.learned_s_limit if headline == "learned" else None,
+                   "svd": svd_precision(headline == "learned")},
         "roofline": line_roof,
         "cpu_baseline": cpu,
         "details": dict(results, median_chi2=float(np.median(head_res[:, 9])),

@@ -79,12 +79,7 @@ typedef struct me_config {
     int      init_mode;        /* ME_INIT_HEURISTIC | ME_INIT_LEARNED       */
     uint64_t seed;             /* Philox4x32-10 key of the restart jumps    */
     int      block_size;       /* 0 = 256; else 64/128/256 (me_ctx_autotune) */
-    /* resident waves per CU.  The FP32 kernel has a build for one wave per
-     * SIMD (4 per CU) and one for two (8 per CU): above 4 selects the
-     * two-wave build.  0 = all that fit of the default build: the two-wave
-     * one for heuristic starts, the one-wave one for learned starts and the
-     * allocation-free call.  The FP64 path has the one-wave build only. */
-    int      waves_per_cu;
+    int      waves_per_cu;     /* 0 = all that fit; resident waves per CU    */
     /* upper damping bound of the learned run only (ME_INIT_LEARNED); 6e-3
      * by default, 0 = lambda_limits[1].  The damping never enters the solve
      * (trial k >= 2 of an iteration solves with the previous trial's U), it
@@ -102,6 +97,18 @@ typedef struct me_config {
      * network's S and may go up to this bound.  Every heuristic run, the
      * fallback's included, keeps [0.6, 1.4] Imax. */
     double   learned_s_limit;
+    /* FP32 path only: the SVD solve in FP32 (1) or FP64 (0); -1 (default)
+     * = FP32 for heuristic starts, FP64 for learned starts.  The FP32 SVD
+     * fits in half the register file, and its kernel build runs two waves
+     * per SIMD (8 per CU) by default, which hides the SVD's dependency
+     * latency: hmi4k plain 21 % faster, converged fraction 82.25 % against
+     * 82.40 %.  The FP64 SVD keeps the learned records nearer the FP64
+     * reference (96 % of hmi4k pixels within 0.1 G / 0.01 deg / 1 m/s of the
+     * FP64 oracle's learned policy against 90 % with the FP32 SVD) at the
+     * same speed: learned maps gain nothing from the second wave, whose
+     * fallback pixels (sequential restarts) then end later.  Results depend
+     * on this setting, never on the launch shape. */
+    int      svd_fp32;
 } me_config;
 
 /* Aggregate counters and optional per-pixel arrays filled by me_invert_*.
@@ -164,8 +171,8 @@ me_status me_ctx_set_region_counts(me_ctx *ctx, int on);
 
 /* Launch-shape autotuner of the persistent inversion kernel: inverts the
  * first n_sample device profiles under every candidate block size (64, 128,
- * 256 lanes) and resident-wave count per CU (one wave per SIMD, and on the
- * FP32 path two) that the occupancy allows (best of two runs each, the context's
+ * 256 lanes) and resident-wave count per CU (one wave per SIMD, and two with
+ * the FP32 SVD) that the occupancy allows (best of two runs each, the context's
  * init mode and precision), keeps the fastest in the context's block_size /
  * waves_per_cu and reports it.  The sample should hold some 64 pixels per
  * resident lane (4M pixels on 256 CUs): a smaller one is timed by its slowest

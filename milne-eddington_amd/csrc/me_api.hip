@@ -168,20 +168,17 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
     st->n_fallback = (int64_t)ct[me::CT_FALLBACK];
 }
 
-// The FP32 inversion kernel has two builds: for one wave per SIMD (a lane's
-// registers are the whole 512-entry file) and for two (256 each; the FP32 SVD
-// fits, the parameters wait in LDS).  A waves_per_cu above one wave per SIMD
-// selects the two-wave build.  By default (waves_per_cu = 0) heuristic starts
-// take the two-wave build and learned starts the one-wave build: the learned
-// map's fallback pixels, whose restarts run one after another, are at the
-// head of the queue, and at half the issue rate per lane they end later than
-// the rest.  The allocation-free path's static scratch holds the one-wave
-// launch's lanes.  The FP64 path has the one-wave build only.
-int invert_build(const me_ctx *c, int learned, bool ext_scratch)
+// The inversion kernel's builds: WPS = 1, one wave per SIMD (a lane's
+// registers are the whole 512-entry file), with the SVD in the path's
+// precision; and on the FP32 path WPS = 2, two waves per SIMD (256 registers
+// each), with the SVD in FP32, which fits there.  svd_fp32 picks the build
+// (by default the FP32 SVD for heuristic starts, the FP64 SVD for learned
+// starts); the launch shape never changes results.
+int invert_build(const me_ctx *c, int learned)
 {
     if (c->cfg.precision == ME_FP64) return 1;
-    if (c->cfg.waves_per_cu > 0) return c->cfg.waves_per_cu > kSimdsPerCU ? 2 : 1;
-    return (learned || ext_scratch) ? 1 : 2;
+    const bool svd32 = c->cfg.svd_fp32 == 1 || (c->cfg.svd_fp32 < 0 && !learned);
+    return svd32 ? 2 : 1;
 }
 
 using InvertKernel = void (*)(const DevCfg *, me::InvArgs);
@@ -246,7 +243,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
                          double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px,
                          const int32_t *order = nullptr, bool records_ready = false)
 {
-    const auto kern = invert_kernel_for<T, TIn>(invert_build(c, learned, S.ext_scratch != nullptr));
+    const auto kern = invert_kernel_for<T, TIn>(invert_build(c, learned));
     int block, grid;
     persistent_grid(c, kern, p1 - p0, block, grid);
     double *scratch = nullptr;
@@ -560,6 +557,7 @@ void me_config_defaults(me_config *c)
     c->seed = 20261200;
     c->learned_lambda_limit = 6e-3;
     c->learned_s_limit = 2.0;
+    c->svd_fp32 = -1;
 }
 
 me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_ctx **out)
@@ -577,7 +575,8 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
     if (!(cfg->learned_s_limit == 0 || cfg->learned_s_limit >= 0.6) || !std::isfinite(cfg->learned_s_limit))
         return ME_EINVAL;
     if (cfg->block_size != 0 && (cfg->block_size < 64 || cfg->block_size > 256 || cfg->block_size % 64))
-        return ME_EINVAL;  // the kernel's LDS counters hold 256 lanes
+        return ME_EINVAL;  // the kernel's LDS columns hold 256 lanes
+    if (cfg->svd_fp32 < -1 || cfg->svd_fp32 > 1) return ME_EINVAL;
     if (cfg->precision != ME_FP64 && cfg->precision != ME_FP32) return ME_EINVAL;
     int nfree = 0;
     for (int p = 0; p < 9; p++) nfree += cfg->fixed[p] ? 0 : 1;
@@ -864,29 +863,27 @@ me_status me_ctx_autotune(me_ctx *c, const void *d_prof, int prof_is_f64, int64_
 {
     if (!c || n <= 0 || !d_prof) return ME_EINVAL;
     if (c->cfg.init_mode == ME_INIT_LEARNED && !c->mlp.trained) return ME_ENOTREADY;
-    // Candidates: one wave per SIMD (the one-wave build) and, on the FP32
-    // path, two (the two-wave build), for each block size whose resident
-    // blocks (registers, LDS) reach that many waves per CU.  Only shapes that
-    // give every SIMD of a CU at least one wave: a shape that leaves SIMDs
-    // idle can only win on a sample whose time is set by its slowest pixels
-    // (a tail of restarted fallback pixels running alone), and then loses by
-    // the idle fraction on a full map.  The kernel's LDS columns are sized for
-    // 256 lanes, so small blocks drop out when LDS caps their residency.
-    auto kern_of = [&](int wps) -> const void * {
-        if (c->cfg.precision == ME_FP64)
-            return prof_is_f64 ? (const void *)invert_kernel_for<double, double>(1)
-                               : (const void *)invert_kernel_for<double, float>(1);
-        return prof_is_f64 ? (const void *)invert_kernel_for<float, double>(wps)
-                           : (const void *)invert_kernel_for<float, float>(wps);
-    };
+    // Candidates: one wave per SIMD and, for the two-wave build, two, for
+    // each block size whose resident blocks (registers, LDS) reach that many
+    // waves per CU.  Only shapes that give every SIMD of a CU at least one
+    // wave: a shape that leaves SIMDs idle can only win on a sample whose time
+    // is set by its slowest pixels (a tail of restarted fallback pixels
+    // running alone), and then loses by the idle fraction on a full map.  The
+    // kernel's LDS columns are sized for 256 lanes, so small blocks drop out
+    // when LDS caps their residency.
+    const int wps = invert_build(c, c->cfg.init_mode == ME_INIT_LEARNED);
+    const void *kern = c->cfg.precision == ME_FP64
+                           ? (prof_is_f64 ? (const void *)invert_kernel_for<double, double>(1)
+                                          : (const void *)invert_kernel_for<double, float>(1))
+                           : (prof_is_f64 ? (const void *)invert_kernel_for<float, double>(wps)
+                                          : (const void *)invert_kernel_for<float, float>(wps));
     struct Cand { int block, waves; };
     std::vector<Cand> cands;
-    const int max_wps = c->cfg.precision == ME_FP64 ? 1 : 2;
-    for (int wps = 1; wps <= max_wps; wps++) {
-        const int waves = wps * kSimdsPerCU;
+    for (int w = 1; w <= wps; w++) {
+        const int waves = w * kSimdsPerCU;
         for (int block : {64, 128, 256}) {
             int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern_of(wps), block, 0) != hipSuccess || per_cu < 1)
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1)
                 per_cu = 1;
             if (waves % (block / 64) == 0 && per_cu * (block / 64) >= waves) cands.push_back({block, waves});
         }

@@ -21,15 +21,47 @@ enum {
     CT_RUNS, CT_ITERS, CT_VALID, CT_CONV, CT_SVDFAIL, CT_FALLBACK, CT_N
 };
 
-// per-lane slots in LDS (one column per lane of the block): the best run's
-// parameters and chi2, the current parameters and the trial parameters
-enum { SL_BEST = 0, SL_CHI_BEST = 9, SL_PAR = 10, SL_PT = 19, SL_LDS_N = 28 };
 // per-lane slots in the scratch buffer (structure of arrays): the gradient,
 // then two 81-entry matrices: the trial's input A and its left factor U
-// alternate between them (mbuf selects the input); then the start parameters
-// of the heuristic runs (Init_Guess) and the learned run's result and chi2,
-// which only the ends of runs read
+// alternate between them (mbuf selects the input); then, for the two-wave
+// build, the start parameters of the heuristic runs (Init_Guess) and the
+// learned run's result and chi2, which only the ends of runs read
 enum { SL_G = 0, SL_MAT = 9, SL_GUESS = 171, SL_LEARN = 180, SL_CHI_LEARN = 189, SL_N = 190 };
+
+// Where a lane's parameters wait between the phase calls, per kernel build
+// (col: the lane's column of the block's LDS slots, stride 256; scr: its
+// scratch slot, stride ns).  One wave per SIMD: the current and trial
+// parameters in registers, the best, start and learned parameters in LDS.
+// Two waves per SIMD (256 registers): the current, trial and best parameters
+// in LDS, so that no register holds them across the calls, and the start and
+// learned parameters in the scratch slot (LDS holds 28 columns of 256 lanes
+// for two blocks per CU, not 38).
+template <int WPS> struct LaneState;
+template <> struct LaneState<1> {
+    enum { BEST = 0, CHI_BEST = 9, GUESS = 10, LEARN = 19, CHI_LEARN = 28, NLDS = 29 };
+    double *col, *scr;
+    int64_t ns;
+    double par_[9], pt_[9];
+    __device__ __forceinline__ double &par(int k) { return par_[k]; }
+    __device__ __forceinline__ double &pt(int k) { return pt_[k]; }
+    __device__ __forceinline__ double &best(int k) { return col[(BEST + k) * 256]; }
+    __device__ __forceinline__ double &chi_best() { return col[CHI_BEST * 256]; }
+    __device__ __forceinline__ double &guess(int k) { return col[(GUESS + k) * 256]; }
+    __device__ __forceinline__ double &learn(int k) { return col[(LEARN + k) * 256]; }
+    __device__ __forceinline__ double &chi_learn() { return col[CHI_LEARN * 256]; }
+};
+template <> struct LaneState<2> {
+    enum { BEST = 0, CHI_BEST = 9, PAR = 10, PT = 19, NLDS = 28 };
+    double *col, *scr;
+    int64_t ns;
+    __device__ __forceinline__ double &par(int k) { return col[(PAR + k) * 256]; }
+    __device__ __forceinline__ double &pt(int k) { return col[(PT + k) * 256]; }
+    __device__ __forceinline__ double &best(int k) { return col[(BEST + k) * 256]; }
+    __device__ __forceinline__ double &chi_best() { return col[CHI_BEST * 256]; }
+    __device__ __forceinline__ double &guess(int k) { return scr[(int64_t)(SL_GUESS + k) * ns]; }
+    __device__ __forceinline__ double &learn(int k) { return scr[(int64_t)(SL_LEARN + k) * ns]; }
+    __device__ __forceinline__ double &chi_learn() { return scr[(int64_t)SL_CHI_LEARN * ns]; }
+};
 // lanes of the allocation-free entry point's static scratch: 256 CUs x 256
 // resident lanes (one 256-lane block per CU: the one-wave build's launch)
 constexpr int64_t kNoAllocLanes = 65536;
@@ -822,28 +854,29 @@ __device__ __noinline__ void pack_free(const DevCfg *__restrict__ Cp, double *__
 }
 
 // WPS: resident waves per SIMD the kernel is built for.  1: each lane has
-// the whole register file (512 entries, VGPRs and AGPRs); 2: half of it,
-// which the FP32 SVD fits in (the FP64 path is built for one wave only).  The
-// __noinline__ phases carry WPS as well, only so that each build calls its
-// own copies, compiled under that build's register budget.
+// the whole register file (512 entries, VGPRs and AGPRs), and the SVD runs
+// in FP64; 2 (FP32 path only): half of it, which the SVD fits in when it
+// runs in FP32.  The __noinline__ phases carry WPS as well,
+// so that each build calls its own copies, compiled under that build's
+// register budget.
 template <typename T, typename TIn, int WPS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS)))
 void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
 {
+    static_assert(WPS == 1 || (WPS == 2 && sizeof(T) == 4), "two-wave build: FP32 path only");
+    using S = typename std::conditional<WPS == 2, float, double>::type;   // the SVD's precision
     CfgRef C = cfg_of(Cp);
     const int nl = C.nl;
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *scr = A.scratch + slot;
     const int64_t NS = A.nslots;
-    // the parameters (current, trial, best of the runs) live in LDS, one
-    // column per lane, so that no register holds them across the phase
-    // calls; the rest of the per-lane state stays in the scratch buffer
-    __shared__ double lsl[SL_LDS_N][256];
-    double *mysl = &lsl[0][threadIdx.x];
-#define LS(k) mysl[(k) * 256]
-#define PAR(k) LS(SL_PAR + (k))
-#define PT(k) LS(SL_PT + (k))
-#define GSL(k) scr[(int64_t)(k) * NS]
+    // the lane's parameters: LDS columns (one per lane), registers and the
+    // scratch slot as the build's register budget allows
+    __shared__ double lsl[LaneState<WPS>::NLDS][256];
+    LaneState<WPS> L;
+    L.col = &lsl[0][threadIdx.x];
+    L.scr = scr;
+    L.ns = NS;
     // event counters: one row per wave in LDS, added to by the wave's sums
     // over its active lanes (blocks of at most 256 lanes)
     __shared__ unsigned long long lct[4][CT_N];
@@ -857,8 +890,8 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
     int32_t px_iters = 0, px_runs = 0, px_trials = 0, px_status = 0;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        PAR(k) = 0.0;
-        PT(k) = 0.0;
+        L.par(k) = 0.0;
+        L.pt(k) = 0.0;
     }
 
     for (;;) {
@@ -935,7 +968,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 bounds_check(C, gs, l7lo, l7hi);
             }
 #pragma unroll
-            for (int k = 0; k < 9; k++) GSL(SL_GUESS + k) = gs[k];
+            for (int k = 0; k < 9; k++) L.guess(k) = gs[k];
             if (A.learned) {
                 // the learned run may bound S wider than Init_Guess (gs[7] is
                 // Imax); the heuristic runs of a fallback restore its bound
@@ -948,7 +981,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 lphase = 0;
             }
 #pragma unroll
-            for (int k = 0; k < 9; k++) PAR(k) = gs[k];
+            for (int k = 0; k < 9; k++) L.par(k) = gs[k];
             ivn = 1. / norm0;
             irun = 0;
             lmchisq = INFINITY;
@@ -975,7 +1008,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             if (__ballot(jl) != 0) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = PAR(k);
+                for (int k = 0; k < 9; k++) pv.v[k] = L.par(k);
                 jr = jac_split<T, TIn, WPS>(Cp, A.prof, jl, base, pv, step7, ivn, scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
                                        scr + (int64_t)SL_G * NS, NS);
                 if (fcount) ct_fadd(wct, jr);
@@ -986,7 +1019,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             if (!split) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = PAR(k);
+                for (int k = 0; k < 9; k++) pv.v[k] = L.par(k);
                 jr = jac_pass<T, TIn, WPS>(Cp, A.prof, base, pv, step7, ivn, mat, scr + (int64_t)SL_G * NS, NS);
                 if (fcount) ct_fadd(wct, jr);
             }
@@ -1000,7 +1033,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 if (C.regul_any) {
                     double pp[9];
 #pragma unroll
-                    for (int k = 0; k < 9; k++) pp[k] = PAR(k);
+                    for (int k = 0; k < 9; k++) pp[k] = L.par(k);
                     const double pen = penalty_of(C, pp);
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
@@ -1013,7 +1046,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 for (int k = 0; k < 9; k++) {
                     const int ip = C.pidx[k];
                     if (C.regul[k] && ip >= 0) {
-                        const double r = PAR(k) - C.rval[k];
+                        const double r = L.par(k) - C.rval[k];
                         scr[(int64_t)(SL_G + ip) * NS] -= (r * C.rw[k]) * ratio;
                         mat[(int64_t)(ip * 10) * NS] += C.rw[k] * ratio;
                     }
@@ -1030,7 +1063,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             // SVD of the input matrix (the normal matrix, or U of the previous
             // rejected trial: H1); U goes to the other slot and becomes the
             // next trial's input.  One call site for every lane of the wave.
-            const Sol9 sv = svd_solve<T, WPS>(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
+            const Sol9 sv = svd_solve<S, WPS>(scr + (int64_t)(SL_MAT + 81 * mbuf) * NS,
                                       scr + (int64_t)(SL_MAT + 81 * (mbuf ^ 1)) * NS, scr + (int64_t)SL_G * NS, NS, C.thr, C.npar);
             ct_inc(wct, CT_SVD);
             ct_add(wct, CT_BK, sv.nbk);
@@ -1041,7 +1074,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             mbuf ^= 1;
             double pt[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) pt[k] = PAR(k);
+            for (int k = 0; k < 9; k++) pt[k] = L.par(k);
             if (C.npar == 9) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) pt[k] += sol[k] * (k == 7 ? step7 : C.step[k]);
@@ -1059,7 +1092,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             }
             bounds_check(C, pt, l7lo, l7hi);
 #pragma unroll
-            for (int k = 0; k < 9; k++) PT(k) = pt[k];
+            for (int k = 0; k < 9; k++) L.pt(k) = pt[k];
         }
         // trial synthesis: shared by the wave on the FP32 path at nl >= 12
         // (trial_split, every lane counts the Faddeeva calls it made), else per
@@ -1069,14 +1102,14 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
             if (__ballot(tl) != 0) {
                 P9 pv;
 #pragma unroll
-                for (int k = 0; k < 9; k++) pv.v[k] = PT(k);
+                for (int k = 0; k < 9; k++) pv.v[k] = L.pt(k);
                 tr = trial_split<T, TIn, WPS>(Cp, A.prof, tl, base, pv, ivn);
                 if (fcount) ct_fadd(wct, tr);
             }
         } else if (tl) {
             P9 pv;
 #pragma unroll
-            for (int k = 0; k < 9; k++) pv.v[k] = PT(k);
+            for (int k = 0; k < 9; k++) pv.v[k] = L.pt(k);
             tr = trial_pass<T, TIn, WPS>(Cp, A.prof, base, pv, ivn);
             if (fcount) ct_fadd(wct, tr);
         }
@@ -1089,13 +1122,13 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 if (C.regul_any) {
                     double pp[9];
 #pragma unroll
-                    for (int k = 0; k < 9; k++) pp[k] = PT(k);
+                    for (int k = 0; k < 9; k++) pp[k] = L.pt(k);
                     const double pen = penalty_of(C, pp);
                     if (pen > 0.1 * lmchisq) ratio = 0.1 * chisq / pen;
                 }
                 const bool conv = ((chisq - chi) / chisq) < C.convg;
 #pragma unroll
-                for (int k = 0; k < 9; k++) PAR(k) = PT(k);
+                for (int k = 0; k < 9; k++) L.par(k) = L.pt(k);
                 chisq = chi;
                 if (conv) {
                     ended = true;
@@ -1122,18 +1155,18 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 // (none when num_run = 0: the learned record is kept)
                 if (crun <= C.crit || C.nruns == 0) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) LS(SL_BEST + k) = PAR(k);
-                    LS(SL_CHI_BEST) = crun;
+                    for (int k = 0; k < 9; k++) L.best(k) = L.par(k);
+                    L.chi_best() = crun;
                     lphase = 0;
                     finish = true;
                 } else {
 #pragma unroll
                     for (int k = 0; k < 9; k++) {
-                        GSL(SL_LEARN + k) = PAR(k);
-                        PAR(k) = GSL(SL_GUESS + k);
+                        L.learn(k) = L.par(k);
+                        L.par(k) = L.guess(k);
                     }
-                    GSL(SL_CHI_LEARN) = crun;
-                    l7hi = PAR(7) * 1.4;   // Init_Guess's bound (Imax * 1.4)
+                    L.chi_learn() = crun;
+                    l7hi = L.par(7) * 1.4;   // Init_Guess's bound (Imax * 1.4)
                     lphase = 2;
                     px_status |= 4;
                     ct_inc(wct, CT_FALLBACK);
@@ -1142,7 +1175,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 }
             } else {
                 bool save = false, brk = false;
-                double cbest = LS(SL_CHI_BEST);
+                double cbest = L.chi_best();
                 if (crun < C.crit) {
                     save = true;
                     brk = (irun >= 4 || irun == C.nruns * 4);
@@ -1153,8 +1186,8 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 }
                 if (save) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) LS(SL_BEST + k) = PAR(k);
-                    LS(SL_CHI_BEST) = crun;
+                    for (int k = 0; k < 9; k++) L.best(k) = L.par(k);
+                    L.chi_best() = crun;
                     cbest = crun;
                 }
                 if (irun >= C.nruns * 2 && cbest < C.crit * 2) brk = true;
@@ -1168,38 +1201,38 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
                 double np[9];
                 if (r < 4) {
 #pragma unroll
-                    for (int k = 0; k < 9; k++) np[k] = LS(SL_BEST + k);
-                    np[2] = GSL(SL_GUESS + 2) + kPi / 4. * r;
+                    for (int k = 0; k < 9; k++) np[k] = L.best(k);
+                    np[2] = L.guess(2) + kPi / 4. * r;
                 } else {
                     double nz[8];
                     gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 0, nz);
                     gauss4(C.seed, A.pix_base + pix, (uint32_t)r, kStreamJump + 1, nz + 4);
-                    np[0] = GSL(SL_GUESS + 0) * (1 + 0.6 * nz[0]);
-                    np[1] = GSL(SL_GUESS + 1) + kPi / 8. * nz[1];
-                    np[2] = GSL(SL_GUESS + 2) + kPi / 4. * (r + 0.3 * nz[2]);
-                    np[3] = GSL(SL_GUESS + 3) + 5. * nz[3];
-                    np[4] = LS(SL_BEST + 4) * (1 + 0.2 * nz[4]);
-                    np[5] = GSL(SL_GUESS + 5) + 0.1 * nz[5];
-                    np[6] = GSL(SL_GUESS + 6) + 5 * nz[6];
-                    np[7] = LS(SL_BEST + 7);
-                    np[8] = GSL(SL_GUESS + 8) + 0.1 * nz[7];
+                    np[0] = L.guess(0) * (1 + 0.6 * nz[0]);
+                    np[1] = L.guess(1) + kPi / 8. * nz[1];
+                    np[2] = L.guess(2) + kPi / 4. * (r + 0.3 * nz[2]);
+                    np[3] = L.guess(3) + 5. * nz[3];
+                    np[4] = L.best(4) * (1 + 0.2 * nz[4]);
+                    np[5] = L.guess(5) + 0.1 * nz[5];
+                    np[6] = L.guess(6) + 5 * nz[6];
+                    np[7] = L.best(7);
+                    np[8] = L.guess(8) + 0.1 * nz[7];
                 }
                 bounds_check(C, np, l7lo, l7hi);
 #pragma unroll
-                for (int k = 0; k < 9; k++) PAR(k) = np[k];
+                for (int k = 0; k < 9; k++) L.par(k) = np[k];
                 phase = PH_JAC_START;
             }
             if (finish) {
                 double best[9];
 #pragma unroll
-                for (int k = 0; k < 9; k++) best[k] = LS(SL_BEST + k);
-                double cbest = LS(SL_CHI_BEST);
+                for (int k = 0; k < 9; k++) best[k] = L.best(k);
+                double cbest = L.chi_best();
                 if (lphase == 2) {
-                    const double cl = GSL(SL_CHI_LEARN);
+                    const double cl = L.chi_learn();
                     if (cl < cbest) {
                         cbest = cl;
 #pragma unroll
-                        for (int k = 0; k < 9; k++) best[k] = GSL(SL_LEARN + k);
+                        for (int k = 0; k < 9; k++) best[k] = L.learn(k);
                     }
                 }
                 if (best[1] > kPi) best[1] -= kPi;
@@ -1229,10 +1262,6 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
 
     // ---------------------------------------------------------- counters
     if (__lane_id() < CT_N && wct[__lane_id()] != 0) atomicAdd(&A.counters[__lane_id()], wct[__lane_id()]);
-#undef LS
-#undef PAR
-#undef PT
-#undef GSL
 }
 
 // forward synthesis, one pixel per thread

@@ -53,6 +53,7 @@ class MeConfig(C.Structure):
         ("waves_per_cu", C.c_int),
         ("learned_lambda_limit", C.c_double),
         ("learned_s_limit", C.c_double),
+        ("svd_fp32", C.c_int),
     ]
 
 
@@ -191,4 +192,5 @@ def reference_defaults() -> MeConfig:
     cfg.waves_per_cu = 0
     cfg.learned_lambda_limit = 6e-3
     cfg.learned_s_limit = 2.0
+    cfg.svd_fp32 = -1
     return cfg

@@ -149,19 +149,22 @@ def test_learned_two_trial_bound_keeps_the_records(learned_case):
     assert np.allclose(qa, qb, rtol=0.02)
 
 
-def test_learned_one_and_two_wave_builds_agree(learned_case):
-    """The FP32 kernel's two builds (one wave per SIMD, the learned default,
-    and two) give the same learned records bit for bit."""
+def test_learned_records_do_not_depend_on_the_launch_shape(learned_case):
+    """Learned records are the same bit for bit under every launch shape of
+    one kernel build: the FP64-SVD build (the learned default on the FP32
+    path) with 256- and 128-lane blocks, and the FP32-SVD build
+    (svd_fp32 = 1) at one and two waves per SIMD."""
     import torch
     w, params, prof, _, _ = learned_case
     d_prof = torch.from_numpy(prof).to("cuda:0")
     d_res = device_tensor((N, 10), torch.float64)
-    recs = []
-    for waves in (4, 8):
-        with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED, block_size=256,
-                           waves_per_cu=waves) as inv:
-            inv.mlp_import(params, w.ranges())
-            inv.invert_device(d_prof.data_ptr(), N, d_res.data_ptr())
-            recs.append(d_res.cpu().numpy())
-    assert np.mean(recs[0][:, 9] > 0) > 0.9
-    np.testing.assert_array_equal(recs[1], recs[0])
+    for svd32, shapes in ((0, ((256, 4), (128, 4))), (1, ((256, 4), (256, 8)))):
+        recs = []
+        for block, waves in shapes:
+            with gpu_inversion("hmi4k", precision=ME_FP32, init_mode=ME_INIT_LEARNED, block_size=block,
+                               waves_per_cu=waves, svd_fp32=svd32) as inv:
+                inv.mlp_import(params, w.ranges())
+                inv.invert_device(d_prof.data_ptr(), N, d_res.data_ptr())
+                recs.append(d_res.cpu().numpy())
+        assert np.mean(recs[0][:, 9] > 0) > 0.9
+        np.testing.assert_array_equal(recs[1], recs[0])

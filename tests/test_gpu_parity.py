@@ -253,7 +253,8 @@ def test_autotuner_picks_a_valid_shape_and_keeps_results():
     """me_ctx_autotune: the chosen block size and resident waves are one of
     the candidates (every SIMD of a CU holds a wave: four or eight per CU),
     stay in the context, and the inversion is the same under every launch
-    shape and in both FP32 builds (one and two waves per SIMD)."""
+    shape (one or two waves per SIMD for the FP32 SVD of the heuristic
+    start)."""
     import torch
     w = get("hmi4k")
     n = 20000
