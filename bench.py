@@ -197,7 +197,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--svd-fp32", type=int, default=-1, choices=[-1, 0, 1],
-                    help="FP32 path: SVD solve in FP32 (1) or FP64 (0); -1: FP32 for heuristic, FP64 for learned starts")
+                    help="FP32 path: SVD solve in FP32 (1) or FP64 (0); -1: FP32 for heuristic starts at nl < 12")
     ap.add_argument("--tune-pixels", type=int, default=1 << 22,
                     help="autotune the launch shape on this many pixels first, about 64 per resident lane "
                          "(0: keep --block/--waves-per-cu)")
@@ -301,7 +301,7 @@ def main():
     def svd_precision(learned):
         if fp64:
             return "fp64"
-        return "fp32" if args.svd_fp32 == 1 or (args.svd_fp32 < 0 and not learned) else "fp64"
+        return "fp32" if args.svd_fp32 == 1 or (args.svd_fp32 < 0 and not learned and w.nl < 12) else "fp64"
 
     def summary(ms, sts, tune=None):
         last = sts[-1]

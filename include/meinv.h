@@ -98,7 +98,7 @@ typedef struct me_config {
      * fallback's included, keeps [0.6, 1.4] Imax. */
     double   learned_s_limit;
     /* FP32 path only: the SVD solve in FP32 (1) or FP64 (0); -1 (default)
-     * = FP32 for heuristic starts, FP64 for learned starts.  The FP32 SVD
+     * = FP32 for heuristic starts at nl < 12, FP64 otherwise.  The FP32 SVD
      * fits in half the register file, and its kernel build runs two waves
      * per SIMD (8 per CU) by default, which hides the SVD's dependency
      * latency: hmi4k plain 21 % faster, converged fraction 82.25 % against
@@ -106,8 +106,10 @@ typedef struct me_config {
      * reference (96 % of hmi4k pixels within 0.1 G / 0.01 deg / 1 m/s of the
      * FP64 oracle's learned policy against 90 % with the FP32 SVD) at the
      * same speed: learned maps gain nothing from the second wave, whose
-     * fallback pixels (sequential restarts) then end later.  Results depend
-     * on this setting, never on the launch shape. */
+     * fallback pixels (sequential restarts) then end later.  At nl >= 12
+     * (the wave-shared passes) the two-wave build was 15 % slower on the
+     * 0.5-1 M-pixel nl = 60 and 112 maps.  Results depend on this setting,
+     * never on the launch shape. */
     int      svd_fp32;
 } me_config;
 

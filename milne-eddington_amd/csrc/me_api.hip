@@ -169,15 +169,16 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
 }
 
 // The inversion kernel's builds: WPS = 1, one wave per SIMD (a lane's
-// registers are the whole 512-entry file), with the SVD in the path's
-// precision; and on the FP32 path WPS = 2, two waves per SIMD (256 registers
-// each), with the SVD in FP32, which fits there.  svd_fp32 picks the build
-// (by default the FP32 SVD for heuristic starts, the FP64 SVD for learned
-// starts); the launch shape never changes results.
+// registers are the whole 512-entry file), with the SVD in FP64; and on the
+// FP32 path WPS = 2, two waves per SIMD (256 registers each), with the SVD in
+// FP32, which fits there.  svd_fp32 picks the build; by default the FP32 SVD
+// for heuristic starts with per-lane passes (nl below the shared passes'
+// threshold), the FP64 SVD otherwise.  The launch shape never changes results.
 int invert_build(const me_ctx *c, int learned)
 {
     if (c->cfg.precision == ME_FP64) return 1;
-    const bool svd32 = c->cfg.svd_fp32 == 1 || (c->cfg.svd_fp32 < 0 && !learned);
+    const bool svd32 =
+        c->cfg.svd_fp32 == 1 || (c->cfg.svd_fp32 < 0 && !learned && c->nl < me::kSharedPassMinNl);
     return svd32 ? 2 : 1;
 }
 

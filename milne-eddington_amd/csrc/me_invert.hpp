@@ -502,6 +502,11 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
     return pos;
 }
 
+// The FP32 path shares its synthesis passes across the wave (jac_split,
+// trial_split) from this many wavelengths on; below it each lane runs its
+// own pixel's passes.
+constexpr int kSharedPassMinNl = 12;
+
 // Fixed wavelength parts of the FP32 path's shared passes: about 14
 // wavelengths each (2 to 8 parts; nl = 60: 4, nl = 112: 8).  Part h covers
 // [h nl / S, (h + 1) nl / S).
@@ -1000,7 +1005,7 @@ void invert_kernel(const DevCfg *__restrict__ Cp, InvArgs A)
         // the FP64 path, each lane runs its own pixel in the reference's
         // summation order.
         const bool jl = phase == PH_JAC_START || phase == PH_JAC;
-        const bool split = sizeof(T) == 4 && nl >= 12;
+        const bool split = sizeof(T) == 4 && nl >= kSharedPassMinNl;
         // Faddeeva call counts: always on the FP64 path, on request on FP32
         const bool fcount = sizeof(T) == 8 || C.fregions != 0;
         PassRes jr{0.0, 0, 0};

@@ -282,27 +282,32 @@ def test_autotuner_picks_a_valid_shape_and_keeps_results():
 
 @pytest.mark.gpu
 def test_split_jacobian_is_independent_of_the_launch_shape():
-    """At nl >= 12 the FP32 path shares each Jacobian pass across the wave
-    (two fixed wavelength halves per pixel, whichever lanes compute them): the
+    """At nl >= 12 the FP32 path shares each synthesis pass across the wave
+    (fixed wavelength parts per pixel, whichever lanes compute them): the
     results must not depend on which pixels share a wave, i.e. on the block
-    size and the number of resident waves."""
+    size and the number of resident waves, in either kernel build (FP64 SVD,
+    the default here, and FP32 SVD at one and two waves per SIMD)."""
     import torch
     w = get("niris64")
     n = 6000
     d_prof = device_tensor((n, 4, w.nl), torch.float32)
     d_res = device_tensor((n, 10), torch.float64)
-    outs = []
-    for block, waves in ((256, 0), (64, 0), (128, 2)):
-        with gpu_inversion("niris64", precision=ME_FP32, block_size=block, waves_per_cu=waves) as inv:
-            if not outs:
-                inv.gen_map_device(w.seed, w.ranges(), n, d_prof.data_ptr(), None, ff_frac=w.ff_frac,
-                                   ff_bmax=w.ff_bmax, sigma=w.sigma)
-            d_res.zero_()
-            inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
-            outs.append(d_res.cpu().numpy())
-    assert np.mean(outs[0][:, 9] > 0) > 0.9
-    np.testing.assert_array_equal(outs[1], outs[0])
-    np.testing.assert_array_equal(outs[2], outs[0])
+    first = True
+    for svd32, shapes in ((-1, ((256, 0), (64, 0), (128, 2))), (1, ((256, 0), (256, 4), (128, 4)))):
+        outs = []
+        for block, waves in shapes:
+            with gpu_inversion("niris64", precision=ME_FP32, block_size=block, waves_per_cu=waves,
+                               svd_fp32=svd32) as inv:
+                if first:
+                    inv.gen_map_device(w.seed, w.ranges(), n, d_prof.data_ptr(), None, ff_frac=w.ff_frac,
+                                       ff_bmax=w.ff_bmax, sigma=w.sigma)
+                    first = False
+                d_res.zero_()
+                inv.invert_device(d_prof.data_ptr(), n, d_res.data_ptr())
+                outs.append(d_res.cpu().numpy())
+        assert np.mean(outs[0][:, 9] > 0) > 0.9
+        np.testing.assert_array_equal(outs[1], outs[0])
+        np.testing.assert_array_equal(outs[2], outs[0])
 
 
 @pytest.mark.parametrize("name,n", [("hinode", 512), ("hmi4k", 2048)])
