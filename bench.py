@@ -15,6 +15,9 @@ Beside the headline (details):
     the workload's: 5 for hmi4k and niris1k, where configurations 1 and 3 put
     Philox restarts);
   * the reference's own precision (FP64 synthesis, --no-fp64 skips it);
+  * the opt-in FP32 SVD (svd_fp32 = 1, the two-waves-per-SIMD kernel build)
+    on the plain map: time and how many records match the default FP64-SVD
+    records within 0.1 G;
   * the learned run under the reference's LM settings (nine trials per LM
     iteration instead of the learned run's two, S bounded to 1.4 Imax instead
     of 2 Imax; --learned-lambda-limit 0 --learned-s-limit 0 make them the
@@ -196,8 +199,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--svd-fp32", type=int, default=-1, choices=[-1, 0, 1],
-                    help="FP32 path: SVD solve in FP32 (1) or FP64 (0); -1: FP32 for heuristic starts at nl < 12")
+    ap.add_argument("--svd-fp32", type=int, default=0, choices=[0, 1],
+                    help="FP32 path: SVD solve in FP64 (0, default) or FP32 (1, the two-wave build)")
     ap.add_argument("--tune-pixels", type=int, default=1 << 22,
                     help="autotune the launch shape on this many pixels first, about 64 per resident lane "
                          "(0: keep --block/--waves-per-cu)")
@@ -299,9 +302,7 @@ def main():
                 "phi_deg": round(float(np.median(dphi) * DEG), 4), "vlos_m_s": round(float(np.median(d[:, 3]) * 1e3), 3)}
 
     def svd_precision(learned):
-        if fp64:
-            return "fp64"
-        return "fp32" if args.svd_fp32 == 1 or (args.svd_fp32 < 0 and not learned and w.nl < 12) else "fp64"
+        return "fp32" if not fp64 and args.svd_fp32 == 1 else "fp64"
 
     def summary(ms, sts, tune=None):
         last = sts[-1]
@@ -335,11 +336,12 @@ def main():
         return train_s, loss
 
     results = {}
-    plain_ms = plain_sts = plain_counted = None
+    plain_ms = plain_sts = plain_counted = plain_res = None
     if args.mode in ("plain", "both"):
         plain_ms, plain_sts, plain_counted, tune = timed(inv, "plain", args.steps, args.warmup, fp64)
+        plain_res = d_res.cpu().numpy()
         results["plain"] = summary(plain_ms, plain_sts, tune)
-        results["plain"]["median_error_vs_truth_converged"] = vs_truth(d_res.cpu().numpy())
+        results["plain"]["median_error_vs_truth_converged"] = vs_truth(plain_res)
     learned_ms = learned_sts = learned_counted = None
     linv = None
     learned_res = None
@@ -409,6 +411,22 @@ def main():
         results[key]["median_error_vs_truth_converged"] = vs_truth(ref_res)
         results["speedup_learned_bounds"] = round(
             results["learned"]["converged_pixels_per_s"] / results[key]["converged_pixels_per_s"], 3)
+        e.close()
+
+    # ---- the opt-in FP32 SVD (svd_fp32 = 1: the two-wave kernel build) on the
+    # plain map, with its records against the default (FP64 SVD) plain
+    # records of the same map
+    if "plain" in results and not fp64 and args.svd_fp32 == 0 and args.restarts:
+        e = Inversion(w.config(**dict(over, svd_fp32=1)), w.wavelengths())
+        ms, sr, _, tune32 = timed(e, "plain, FP32 SVD", 1, 1, fp64, count=False)
+        s32 = d_res.cpu().numpy()
+        key = "plain_svd_fp32"
+        results[key] = summary(ms, sr, tune32)
+        both = (s32[:, 9] > 0) & (plain_res[:, 9] > 0)
+        dB = np.abs(s32[both, 0] - plain_res[both, 0])
+        results[key]["same_as_fp64_svd_within_0.1G"] = round(float(np.mean(dB <= 0.1)), 4)
+        results[key]["median_error_vs_truth_converged"] = vs_truth(s32)
+        results[key]["speedup_vs_fp64_svd"] = round(results["plain"]["ms_per_step"] / ms, 3)
         e.close()
 
     # ---- other restart budgets: num_run 1 (no Gaussian jumps), 5 and 10

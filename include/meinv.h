@@ -31,7 +31,8 @@ typedef int me_status;
 #define ME_ENOTREADY 4   /* learned initialiser requested but not trained/loaded */
 
 /* precision of synthesis, Faddeeva, Jacobian and normal matrix.  The 9x9 SVD
- * solve and the chi2 accept/reject test are FP64 in both. */
+ * solve is FP64 (unless me_config.svd_fp32 opts into FP32 on the FP32 path)
+ * and the chi2 accept/reject test FP64 in both. */
 #define ME_FP64 0
 #define ME_FP32 1
 
@@ -97,19 +98,17 @@ typedef struct me_config {
      * network's S and may go up to this bound.  Every heuristic run, the
      * fallback's included, keeps [0.6, 1.4] Imax. */
     double   learned_s_limit;
-    /* FP32 path only: the SVD solve in FP32 (1) or FP64 (0); -1 (default)
-     * = FP32 for heuristic starts at nl < 12, FP64 otherwise.  The FP32 SVD
-     * fits in half the register file, and its kernel build runs two waves
-     * per SIMD (8 per CU) by default, which hides the SVD's dependency
-     * latency: hmi4k plain 21 % faster, converged fraction 82.25 % against
-     * 82.40 %.  The FP64 SVD keeps the learned records nearer the FP64
-     * reference (96 % of hmi4k pixels within 0.1 G / 0.01 deg / 1 m/s of the
-     * FP64 oracle's learned policy against 90 % with the FP32 SVD) at the
-     * same speed: learned maps gain nothing from the second wave, whose
-     * fallback pixels (sequential restarts) then end later.  At nl >= 12
-     * (the wave-shared passes) the two-wave build was 15 % slower on the
-     * 0.5-1 M-pixel nl = 60 and 112 maps.  Results depend on this setting,
-     * never on the launch shape. */
+    /* FP32 path only: 0 (default) keeps the 9x9 SVD solve in FP64, as the
+     * normal matrix's conditioning needs; 1 runs it in FP32, an opt-in
+     * throughput mode.  The FP32 SVD fits in half the register file, and its
+     * kernel build runs two waves per SIMD (8 per CU), which hides the SVD's
+     * dependency latency: hmi4k plain 21 % faster with the converged
+     * fraction kept (82.25 % against 82.40 %), but fewer pixels at the FP64
+     * solution: against the FP64 oracle, 77 % of 2048 hmi4k plain pixels
+     * within 0.1 G / 0.01 deg / 1 m/s against 87 % with the FP64 SVD, and
+     * 90 % of learned pixels against 96 %.  Learned maps gain no speed from
+     * it, and the nl = 60 and 112 maps lose 15 %.  Results depend on this
+     * setting, never on the launch shape. */
     int      svd_fp32;
 } me_config;
 

@@ -170,16 +170,12 @@ void fill_stats(me_stats *st, const unsigned long long *ct, int64_t n)
 
 // The inversion kernel's builds: WPS = 1, one wave per SIMD (a lane's
 // registers are the whole 512-entry file), with the SVD in FP64; and on the
-// FP32 path WPS = 2, two waves per SIMD (256 registers each), with the SVD in
-// FP32, which fits there.  svd_fp32 picks the build; by default the FP32 SVD
-// for heuristic starts with per-lane passes (nl below the shared passes'
-// threshold), the FP64 SVD otherwise.  The launch shape never changes results.
-int invert_build(const me_ctx *c, int learned)
+// FP32 path, when svd_fp32 opts into the FP32 SVD, WPS = 2, two waves per
+// SIMD (256 registers each), which the FP32 SVD fits in.  The launch shape
+// never changes results.
+int invert_build(const me_ctx *c)
 {
-    if (c->cfg.precision == ME_FP64) return 1;
-    const bool svd32 =
-        c->cfg.svd_fp32 == 1 || (c->cfg.svd_fp32 < 0 && !learned && c->nl < me::kSharedPassMinNl);
-    return svd32 ? 2 : 1;
+    return c->cfg.precision == ME_FP32 && c->cfg.svd_fp32 == 1 ? 2 : 1;
 }
 
 using InvertKernel = void (*)(const DevCfg *, me::InvArgs);
@@ -244,7 +240,7 @@ me_status enqueue_invert(me_ctx *c, const Site &S, const void *d_prof, int64_t p
                          double *d_res, double *d_fitpar, const double *d_init, int learned, int32_t *d_px,
                          const int32_t *order = nullptr, bool records_ready = false)
 {
-    const auto kern = invert_kernel_for<T, TIn>(invert_build(c, learned));
+    const auto kern = invert_kernel_for<T, TIn>(invert_build(c));
     int block, grid;
     persistent_grid(c, kern, p1 - p0, block, grid);
     double *scratch = nullptr;
@@ -558,7 +554,7 @@ void me_config_defaults(me_config *c)
     c->seed = 20261200;
     c->learned_lambda_limit = 6e-3;
     c->learned_s_limit = 2.0;
-    c->svd_fp32 = -1;
+    c->svd_fp32 = 0;
 }
 
 me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_ctx **out)
@@ -577,7 +573,7 @@ me_status me_ctx_create(const me_config *cfg, const double *lambda, int nl, me_c
         return ME_EINVAL;
     if (cfg->block_size != 0 && (cfg->block_size < 64 || cfg->block_size > 256 || cfg->block_size % 64))
         return ME_EINVAL;  // the kernel's LDS columns hold 256 lanes
-    if (cfg->svd_fp32 < -1 || cfg->svd_fp32 > 1) return ME_EINVAL;
+    if (cfg->svd_fp32 != 0 && cfg->svd_fp32 != 1) return ME_EINVAL;
     if (cfg->precision != ME_FP64 && cfg->precision != ME_FP32) return ME_EINVAL;
     int nfree = 0;
     for (int p = 0; p < 9; p++) nfree += cfg->fixed[p] ? 0 : 1;
@@ -872,7 +868,7 @@ me_status me_ctx_autotune(me_ctx *c, const void *d_prof, int prof_is_f64, int64_
     // running alone), and then loses by the idle fraction on a full map.  The
     // kernel's LDS columns are sized for 256 lanes, so small blocks drop out
     // when LDS caps their residency.
-    const int wps = invert_build(c, c->cfg.init_mode == ME_INIT_LEARNED);
+    const int wps = invert_build(c);
     const void *kern = c->cfg.precision == ME_FP64
                            ? (prof_is_f64 ? (const void *)invert_kernel_for<double, double>(1)
                                           : (const void *)invert_kernel_for<double, float>(1))

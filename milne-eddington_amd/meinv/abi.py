@@ -192,5 +192,5 @@ def reference_defaults() -> MeConfig:
     cfg.waves_per_cu = 0
     cfg.learned_lambda_limit = 6e-3
     cfg.learned_s_limit = 2.0
-    cfg.svd_fp32 = -1
+    cfg.svd_fp32 = 0
     return cfg
