@@ -408,9 +408,12 @@ __device__ __forceinline__ void mlp_hidden_layer(const bf16x8 *__restrict__ sW, 
         // one output tile at a time: keeps the scheduler from hoisting every
         // tile's LDS reads into registers
         __builtin_amdgcn_sched_barrier(0);
+        // the accumulators start at the bias (a lane's four rows are output
+        // channels 16 t + 4 g .. + 3 for every pixel column)
+        const f32x4 b = f32x4{sb[16 * t + 4 * g], sb[16 * t + 4 * g + 1], sb[16 * t + 4 * g + 2], sb[16 * t + 4 * g + 3]};
         f32x4 acc[MI_C];
 #pragma unroll
-        for (int c = 0; c < MI_C; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < MI_C; c++) acc[c] = b;
 #pragma unroll
         for (int kb = 0; kb < KB; kb++) {
             const bf16x8 a = sW[(t * KB + kb) * 64 + lane];
@@ -420,7 +423,7 @@ __device__ __forceinline__ void mlp_hidden_layer(const bf16x8 *__restrict__ sW, 
 #pragma unroll
         for (int c = 0; c < MI_C; c++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) nxt[c][t >> 1][4 * (t & 1) + v] = to_bf16(gelu(acc[c][v] + sb[16 * t + 4 * g + v]));
+            for (int v = 0; v < 4; v++) nxt[c][t >> 1][4 * (t & 1) + v] = to_bf16(gelu(acc[c][v]));
     }
 #pragma unroll
     for (int c = 0; c < MI_C; c++)
@@ -501,13 +504,11 @@ __global__ __launch_bounds__(MI_T) void mlp_infer_kernel(const TIn *__restrict__
                 // and decode
 #pragma unroll
                 for (int c = 0; c < MI_C; c++) {
-                    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+                    f32x4 acc = f32x4{sb[4 * g], sb[4 * g + 1], sb[4 * g + 2], sb[4 * g + 3]};   // starts at the bias
 #pragma unroll
                     for (int kb = 0; kb < 8; kb++)
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sW[kb * 64 + lane], act[c][kb], acc, 0, 0, 0);
                     float z[16];
-#pragma unroll
-                    for (int v = 0; v < 4; v++) acc[v] += sb[4 * g + v];
 #pragma unroll
                     for (int j = 0; j < 12; j++) z[j] = __shfl(acc[j & 3], (j >> 2) * 16 + col);
                     const int64_t p = base + (wave * MI_C + c) * 16 + col;

@@ -373,6 +373,35 @@ def test_fp32_svd_against_fp64_svd_and_the_oracle(name, n):
     assert abs(conv[1] - conv[0]) <= 0.01
 
 
+@pytest.mark.parametrize("name", ["hmi4k", "niris64"])
+def test_fp32_svd_build_with_fixed_parameters_regularisation_and_restarts(name):
+    """The FP32 SVD's two-wave build on the paths the learned and plain
+    defaults leave aside there: fixed parameters (the packed system), the
+    theta regularisation and num_run = 5 (restarts from the start parameters
+    and the best run, Gaussian jumps from run 4 on, which that build keeps in
+    the scratch slot and LDS).  Against the FP64 SVD's build and the FP64
+    oracle on the same pixels: converged fractions within 3 points."""
+    n = 1024
+    over = dict(num_run=5, fixed=(0, 0, 0, 0, 0, 1, 0, 0, 1), regul_theta=True)
+    w, _, prof, _ = oracle_map(name, n)
+    orc = O.Oracle(w.config(**over), w.wavelengths())
+    ref = orc.invert(prof.astype(np.float64), nthreads=8)
+    conv_ref = np.mean((ref["res"][:, 9] > 0) & (ref["res"][:, 9] <= w.chisq_criteria))
+    conv, runs = {}, {}
+    for svd32 in (0, 1):
+        with gpu_inversion(name, precision=ME_FP32, svd_fp32=svd32, **over) as inv:
+            out = inv.invert(prof, per_pixel=True)
+        r = out["res"]
+        conv[svd32] = np.mean((r[:, 9] > 0) & (r[:, 9] <= w.chisq_criteria))
+        runs[svd32] = out["stats"].per_pixel["runs"].max()
+        assert out["stats"].n_svd_fail == 0
+    print(f"{name} fixed (5, 8) + theta regularisation + num_run 5: converged {conv[1]:.4f} (FP32 SVD) / "
+          f"{conv[0]:.4f} (FP64 SVD) / {conv_ref:.4f} (oracle); most runs per pixel {runs[1]} / {runs[0]}")
+    assert runs[1] >= 5
+    assert abs(conv[1] - conv[0]) <= 0.03
+    assert abs(conv[1] - conv_ref) <= 0.03
+
+
 def test_synthesis_fp64_special_lines(golden):
     """Faddeeva's special lines on the device (Src/FADDEEVA.c:461-468): a = 0
     puts every call on y = 0 (exp(-nu^2), Dawson); B = 0 and v_los = 0 on the
