@@ -19,7 +19,7 @@ Beside the headline (details):
     on the plain map: time and how many records match the default FP64-SVD
     records within 0.1 G;
   * the learned run under the reference's LM settings (nine trials per LM
-    iteration instead of the learned run's two, S bounded to 1.4 Imax instead
+    iteration instead of the learned run's one, S bounded to 1.4 Imax instead
     of 2 Imax; --learned-lambda-limit 0 --learned-s-limit 0 make them the
     headline): time, trials, converged fraction, identical records, chi2
     quantiles;
@@ -215,8 +215,8 @@ def main():
                     help="also time plain and learned at the other num_run of 1, 5, 10 (0 skips)")
     ap.add_argument("--oracle-pixels", type=int, default=16384,
                     help="pixels of the learned-vs-oracle check (0 skips it)")
-    ap.add_argument("--learned-lambda-limit", type=float, default=6e-3,
-                    help="damping bound of the learned run (6e-3: two trials per iteration; 0: lambda_limits[1], "
+    ap.add_argument("--learned-lambda-limit", type=float, default=4e-3,
+                    help="damping bound of the learned run (4e-3: one trial per iteration, 6e-3: two; 0: lambda_limits[1], "
                          "the reference's nine); the fallback's heuristic runs keep the reference bound")
     ap.add_argument("--learned-s-limit", type=float, default=2.0,
                     help="upper bound of S in the learned run in units of the profile's I maximum (0: "

@@ -81,14 +81,15 @@ typedef struct me_config {
     uint64_t seed;             /* Philox4x32-10 key of the restart jumps    */
     int      block_size;       /* 0 = 256; else 64/128/256 (me_ctx_autotune) */
     int      waves_per_cu;     /* 0 = all that fit; resident waves per CU    */
-    /* upper damping bound of the learned run only (ME_INIT_LEARNED); 6e-3
+    /* upper damping bound of the learned run only (ME_INIT_LEARNED); 4e-3
      * by default, 0 = lambda_limits[1].  The damping never enters the solve
      * (trial k >= 2 of an iteration solves with the previous trial's U), it
      * only bounds the trials of one iteration: damp * lambda_reject^r > bound
-     * ends the run after r rejected trials, so 6e-3 allows two (the second
-     * solves with the normal matrix's U; later ones would solve with the U of
-     * an orthogonal matrix, which rounding decides).  The heuristic runs of
-     * the fallback always use lambda_limits[1]. */
+     * ends the run after r rejected trials, so with the default damping
+     * (1e-3, x5 per rejection) 4e-3 allows one trial per iteration, 6e-3 two
+     * (the second solves with the normal matrix's U; later ones would solve
+     * with the U of an orthogonal matrix, which rounding decides).  The
+     * heuristic runs of the fallback always use lambda_limits[1]. */
     double   learned_lambda_limit;
     /* upper bound of S in the learned run, in units of the profile's I
      * maximum; 2 by default, 0 keeps Init_Guess's 1.4.  Init_Guess starts S at

@@ -552,7 +552,7 @@ void me_config_defaults(me_config *c)
     c->precision = ME_FP64;
     c->init_mode = ME_INIT_HEURISTIC;
     c->seed = 20261200;
-    c->learned_lambda_limit = 6e-3;
+    c->learned_lambda_limit = 4e-3;
     c->learned_s_limit = 2.0;
     c->svd_fp32 = 0;
 }

@@ -29,7 +29,7 @@
 // [0, pi], S over [0.5, 1.2] relative to the pixel's continuum, 30 % of the
 // pixels field-free below 50 G, noise 1e-3 S).  --learned-lambda-limit X
 // bounds the damping of the learned run (me_config.learned_lambda_limit: the
-// library's 6e-3 allows two trials per iteration, 0 takes lambda_limits[1]);
+// library's 4e-3 allows one trial per iteration, 0 takes lambda_limits[1]);
 // --learned-s-limit F bounds S in the learned run to F times the profile's I
 // maximum (me_config.learned_s_limit: 2 by default, 0 keeps Init_Guess's 1.4).
 #include <sys/time.h>

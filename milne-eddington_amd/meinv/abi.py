@@ -190,7 +190,7 @@ def reference_defaults() -> MeConfig:
     cfg.seed = 20261200
     cfg.block_size = 0
     cfg.waves_per_cu = 0
-    cfg.learned_lambda_limit = 6e-3
+    cfg.learned_lambda_limit = 4e-3
     cfg.learned_s_limit = 2.0
     cfg.svd_fp32 = 0
     return cfg

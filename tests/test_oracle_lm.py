@@ -303,6 +303,35 @@ def test_learned_run_damping_bound_applies_to_the_learned_run_only():
             assert np.any(lt > 2 * (li + 1))
 
 
+def test_learned_run_default_bound_is_one_trial_per_iteration():
+    """The default learned_lambda_limit (4e-3: 1e-3 * 5 > 4e-3) ends an
+    iteration of the learned run at its first rejected trial: a pixel without
+    fallback runs exactly one trial per iteration it starts (iters accepted
+    ones plus the last), fewer trials than under 6e-3, and the records of
+    pixels that converge within the bound agree with the two-trial run's
+    where both accept only first trials."""
+    w = get("hmi4k")
+    cfg = w.config(num_run=1)
+    assert abs(cfg.learned_lambda_limit - 4e-3) < 1e-15
+    prof, truth = O.Oracle(cfg, w.wavelengths()).gen_map(w.seed, w.ranges(), 200, ff_frac=w.ff_frac,
+                                                           ff_bmax=w.ff_bmax, sigma=w.sigma)
+    prof = prof.astype(np.float64)
+    start = truth.copy()
+    start[:, 0] *= 1.3
+    start[:, 3] += 0.3
+    one = O.Oracle(cfg, w.wavelengths()).invert(prof, init=start, learned=True, nthreads=8)
+    two = O.Oracle(w.config(num_run=1, learned_lambda_limit=6e-3), w.wavelengths()).invert(
+        prof, init=start, learned=True, nthreads=8)
+    direct = ((one["status"] & 4) == 0) & (one["runs"] > 0)
+    assert direct.mean() > 0.5
+    assert np.all(one["trials"][direct] == one["iters"][direct] + 1)
+    assert one["trials"].sum() < two["trials"].sum()
+    # a two-trial run that never rejected a trial took the same steps
+    same = direct & ((two["status"] & 4) == 0) & (two["trials"] == two["iters"] + 1)
+    assert same.any()
+    np.testing.assert_array_equal(one["res"][same], two["res"][same])
+
+
 def test_learned_run_s_bound_reaches_lines_without_continuum():
     """Init_Guess bounds S to [0.6, 1.4] Imax (Src/ME_SOLVER.c:662-666): a line
     that never reaches the continuum in the window (truth S above 1.4 Imax)
