@@ -72,15 +72,17 @@ enum { EPI_FWD_GELU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_GELU = 2, EPI_F32 = 3 };
 // ragged edge.
 enum { LAY_ANY = 0, LAY_K = 1, LAY_MN = 2 };
 
+// A thread's share of one 64x32 operand tile: 8 values, loaded from global
+// memory (tile_load) and written to the LDS tile (tile_store).  LAY_K and
+// LAY_ANY: row r = tid / 4 (m or n), k = 8 (tid % 4) .. +7; LAY_MN: k = tid / 8,
+// rows 8 (tid % 8) .. +7.
 template <int LAY>
-__device__ __forceinline__ void stage_tile(__bf16 (*sT)[40], const __bf16 *__restrict__ P, int64_t s_mn, int64_t s_k,
-                                           int mn0, int MN, int k0, int kend, int tid)
+__device__ __forceinline__ bf16x8 tile_load(const __bf16 *__restrict__ P, int64_t s_mn, int64_t s_k, int mn0, int MN,
+                                            int k0, int kend, int tid)
 {
+    bf16x8 v;
     if (LAY == LAY_K) {
-        // thread: row r (m or n), k group kk .. kk+7
-        const int r = tid >> 2, kk = (tid & 3) * 8;
-        const int mn = mn0 + r, k = k0 + kk;
-        bf16x8 v;
+        const int mn = mn0 + (tid >> 2), k = k0 + (tid & 3) * 8;
         if (mn < MN && k + 7 < kend) {
             v = *(const bf16x8 *)(P + (int64_t)mn * s_mn + k);
         } else {
@@ -88,12 +90,8 @@ __device__ __forceinline__ void stage_tile(__bf16 (*sT)[40], const __bf16 *__res
             for (int q = 0; q < 8; q++)
                 v[q] = (mn < MN && k + q < kend) ? P[(int64_t)mn * s_mn + (int64_t)(k + q) * s_k] : to_bf16(0.f);
         }
-        *(bf16x8 *)&sT[r][kk] = v;
     } else if (LAY == LAY_MN) {
-        // thread: k index kk, rows r .. r+7
-        const int kk = tid >> 3, r = (tid & 7) * 8;
-        const int mn = mn0 + r, k = k0 + kk;
-        bf16x8 v;
+        const int mn = mn0 + (tid & 7) * 8, k = k0 + (tid >> 3);
         if (mn + 7 < MN && k < kend) {
             v = *(const bf16x8 *)(P + (int64_t)k * s_k + mn);
         } else {
@@ -101,18 +99,30 @@ __device__ __forceinline__ void stage_tile(__bf16 (*sT)[40], const __bf16 *__res
             for (int q = 0; q < 8; q++)
                 v[q] = (mn + q < MN && k < kend) ? P[(int64_t)(mn + q) * s_mn + (int64_t)k * s_k] : to_bf16(0.f);
         }
+    } else {
+        const int mn = mn0 + (tid >> 2), k = k0 + (tid & 3) * 8;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            v[q] = (mn < MN && k + q < kend) ? P[(int64_t)mn * s_mn + (int64_t)(k + q) * s_k] : to_bf16(0.f);
+    }
+    return v;
+}
+
+template <int LAY>
+__device__ __forceinline__ void tile_store(__bf16 (*sT)[40], const bf16x8 &v, int tid)
+{
+    if (LAY == LAY_MN) {
+        const int kk = tid >> 3, r = (tid & 7) * 8;
 #pragma unroll
         for (int q = 0; q < 8; q++) sT[r + q][kk] = v[q];
     } else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int idx = tid * 8 + e;
-            const int r = idx >> 5, kk = idx & 31;
-            const int mn = mn0 + r, k = k0 + kk;
-            sT[r][kk] = (mn < MN && k < kend) ? P[(int64_t)mn * s_mn + (int64_t)k * s_k] : to_bf16(0.f);
-        }
+        *(bf16x8 *)&sT[tid >> 2][(tid & 3) * 8] = v;
     }
 }
+
+// k slabs of 32 staged per barrier pair: every global load of the KS slabs
+// is in flight at once, and a GEMM with K <= 32 KS takes one round trip
+constexpr int GEMM_KS = 4;
 
 template <int EPI, int ALAY = LAY_ANY, int BLAY = LAY_ANY>
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
@@ -121,8 +131,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
                                                  const float *__restrict__ bias, const float *__restrict__ aux,
                                                  float scale, int64_t zs)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 sA[64][32 + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[64][32 + 8];  // stored transposed: sB[n][k]
+    __shared__ __attribute__((aligned(16))) __bf16 sA[GEMM_KS][64][32 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[GEMM_KS][64][32 + 8];  // stored transposed: sB[n][k]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
     // split K over gridDim.z (weight gradients: K is the batch)
@@ -132,18 +142,38 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16 *__restrict__ A, i
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        // 64x32 tiles of A and B^T
-        stage_tile<ALAY>(sA, A, sam, sak, m0, M, k0, kend, tid);
-        stage_tile<BLAY>(sB, B, sbn, sbk, n0, N, k0, kend, tid);
+    for (int k0 = kbeg; k0 < kend; k0 += 32 * GEMM_KS) {
+        // GEMM_KS 64x32 tiles of A and B^T (a slab past kend is skipped, as
+        // the whole block sees); the MFMAs then run in k order, so every
+        // output is the same chain of products whatever GEMM_KS is
+        bf16x8 va[GEMM_KS], vb[GEMM_KS];
+#pragma unroll
+        for (int q = 0; q < GEMM_KS; q++) {
+            if (k0 + 32 * q < kend) {
+                va[q] = tile_load<ALAY>(A, sam, sak, m0, M, k0 + 32 * q, kend, tid);
+                vb[q] = tile_load<BLAY>(B, sbn, sbk, n0, N, k0 + 32 * q, kend, tid);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GEMM_KS; q++) {
+            if (k0 + 32 * q < kend) {
+                tile_store<ALAY>(sA[q], va[q], tid);
+                tile_store<BLAY>(sB[q], vb[q], tid);
+            }
+        }
         __syncthreads();
         // lane l: row/col (l % 16), k group 8*(l / 16) .. +7
         const int rr = lane & 15, kg = (lane >> 4) * 8;
-        const bf16x8 a = *(const bf16x8 *)&sA[wave * 16 + rr][kg];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const bf16x8 b = *(const bf16x8 *)&sB[t * 16 + rr][kg];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        for (int q = 0; q < GEMM_KS; q++) {
+            if (k0 + 32 * q < kend) {
+                const bf16x8 a = *(const bf16x8 *)&sA[q][wave * 16 + rr][kg];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const bf16x8 b = *(const bf16x8 *)&sB[q][t * 16 + rr][kg];
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
         }
         __syncthreads();
     }
