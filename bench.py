@@ -206,7 +206,7 @@ def main():
                          "(0: keep --block/--waves-per-cu)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"])
     ap.add_argument("--train-samples", type=int, default=8 * 1024 * 1024,
-                    help="device-synthesised training profiles (8M x 24 epochs: 6.6 s on hmi4k)")
+                    help="device-synthesised training profiles (8M x 24 epochs: 6.2 s on hmi4k)")
     ap.add_argument("--train-epochs", type=int, default=24)
     ap.add_argument("--mlp-in", default="", help="use this saved network instead of training one")
     ap.add_argument("--save-mlp", default="", help="write the trained network to this file")
