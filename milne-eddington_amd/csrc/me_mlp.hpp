@@ -347,14 +347,17 @@ __global__ __launch_bounds__(256) void sum_partials(const float *__restrict__ pa
     out[i] = s;
 }
 
-// Adam on FP32 masters, then the bf16 copy used by the GEMMs
+// Adam on FP32 masters, then the bf16 copy used by the GEMMs.  The gradient
+// of entry i is the sum of its nz chunk partials part[z][i], added in chunk
+// order exactly as sum_partials does (one launch fewer per training step)
 __global__ void adam_step(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
-                          const float *__restrict__ g, __bf16 *__restrict__ pb, int64_t n, float lr, float b1, float b2,
-                          float eps, float c1, float c2)
+                          const float *__restrict__ part, int nz, __bf16 *__restrict__ pb, int64_t n, float lr, float b1,
+                          float b2, float eps, float c1, float c2)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i];
+    float gi = part[i];
+    for (int z = 1; z < nz; z++) gi += part[(int64_t)z * n + i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
@@ -726,15 +729,20 @@ inline void mlp_forward(MlpState &m, hipStream_t s, int rows, const __bf16 *X = 
     }
 }
 
-// backward pass (loss gradient already in m.D[last]); gradients into m.G
-inline void mlp_backward(MlpState &m, hipStream_t s, int rows, const __bf16 *X = nullptr)
+// K chunks of a weight-gradient GEMM (and row chunks of the bias sums) for a
+// batch of `rows`
+inline int mlp_ksplit(int rows) { return rows >= 4096 ? MLP_KSPLIT : (rows >= 1024 ? 8 : 1); }
+
+// backward pass (loss gradient already in m.D[last]); gradients into m.G, or
+// with sum = false only their chunk partials into m.Gp (adam_step adds them)
+inline void mlp_backward(MlpState &m, hipStream_t s, int rows, const __bf16 *X = nullptr, bool sum = true)
 {
     if (!X) X = m.X;
     // every gradient entry is the sum of ksplit chunk partials (weights: the
     // K chunks of the GEMM; biases: row chunks), which sit in Gp at stride np
     // at the entry's own offset; one sum_partials adds them all in chunk
     // order at the end, so the gradient is the same in every run
-    const int ksplit = rows >= 4096 ? MLP_KSPLIT : (rows >= 1024 ? 8 : 1);
+    const int ksplit = mlp_ksplit(rows);
     for (int l = MLP_NL - 1; l >= 0; l--) {
         const int din = m.dims[l], dout = m.dims[l + 1];
         const __bf16 *act = l == 0 ? X : m.A[l - 1];
@@ -756,8 +764,9 @@ inline void mlp_backward(MlpState &m, hipStream_t s, int rows, const __bf16 *X =
                                (const float *)nullptr, (const float *)m.Z[l - 1], 1.0f, (int64_t)0);
         }
     }
-    hipLaunchKernelGGL(sum_partials, dim3((unsigned)((m.np + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
-                       ksplit, m.np, m.G);
+    if (sum)
+        hipLaunchKernelGGL(sum_partials, dim3((unsigned)((m.np + 255) / 256)), dim3(256), 0, s, (const float *)m.Gp,
+                           ksplit, m.np, m.G);
 }
 
 // the batch loss of the last mlp_loss launch (its wave partials added in
@@ -819,13 +828,13 @@ me_status mlp_train_impl(MlpState &m, const DevCfg &C, hipStream_t s, uint64_t s
         mlp_forward(m, s, batch, Xb);
         hipLaunchKernelGGL(mlp_loss, dim3((batch + 255) / 256), dim3(256), 0, s, (const float *)m.Z[MLP_NL - 1],
                            (const float *)(m.Yall + r0 * MLP_OUTP), batch, m.D[MLP_NL - 1], m.loss);
-        mlp_backward(m, s, batch, Xb);
+        mlp_backward(m, s, batch, Xb, false);
         // cosine decay to 5% of lr
         const double frac = (double)step / (double)total;
         const float lrs = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(kPi * frac))));
         const float c1 = 1.f - powf(b1, (float)(step + 1)), c2 = 1.f - powf(b2, (float)(step + 1));
-        hipLaunchKernelGGL(adam_step, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Mo, m.Vo, (const float *)m.G,
-                           m.Pb, m.np, lrs, b1, b2, 1e-8f, c1, c2);
+        hipLaunchKernelGGL(adam_step, dim3((m.np + 255) / 256), dim3(256), 0, s, m.P, m.Mo, m.Vo, (const float *)m.Gp,
+                           mlp_ksplit(batch), m.Pb, m.np, lrs, b1, b2, 1e-8f, c1, c2);
         if (step == total - 1) {
             float lh = 0;
             if (mlp_loss_total(m, s, batch, &lh) != hipSuccess) return ME_EDEVICE;
